@@ -1,0 +1,201 @@
+/*
+ * bkma_amd.h -- C ABI of the MI355X (gfx950) BKMA library.
+ *
+ * "BKMA" = batched kernels with memory allocations: in-place line operators
+ * over a row-major [n0][n1][n2] array, each (i0, :, i2) line updated by a
+ * solver functor with a scratch (LDS or global) holding the new values.
+ *
+ * Conventions
+ *  - Every pointer to array data is DEVICE memory allocated by the caller
+ *    (hipMalloc, torch tensors, ...).  Nothing here takes ownership.
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *    Work is enqueued asynchronously; the caller synchronises.  Entry points
+ *    that return host values (validators, norms) synchronise `stream`.
+ *  - Errors are status codes; bkma_last_error() returns the message of the
+ *    last failing call on the calling thread.  The C++ facade rethrows them
+ *    with the reference's exception types and texts.
+ *  - Not thread-safe per stream: calls that share a stream must not overlap.
+ *  - Compute paths never fall back to the CPU: with no usable gfx950 device
+ *    they return BKMA_ERR_DEVICE.
+ */
+#ifndef BKMA_AMD_H
+#define BKMA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BKMA_AMD_ABI_VERSION 1
+
+typedef enum bkma_status {
+    BKMA_OK = 0,
+    BKMA_ERR_INVALID_ARGUMENT = 1, /* reference: std::invalid_argument */
+    BKMA_ERR_RUNTIME = 2,          /* reference: std::runtime_error */
+    BKMA_ERR_DEVICE = 3,           /* HIP runtime failure (reference: sycl::exception) */
+    BKMA_ERR_UNSUPPORTED = 4,      /* shape/dtype combination this build does not run */
+    BKMA_ERR_OUT_OF_MEMORY = 5
+} bkma_status;
+
+/* BkmaImpl, src/core/bkma_tools.hpp:9-13 */
+typedef enum bkma_impl {
+    BKMA_IMPL_BASIC_RANGE = 0,
+    BKMA_IMPL_ND_RANGE = 1,
+    BKMA_IMPL_ADAPTIVE_WG = 2
+} bkma_impl;
+
+/* MemorySpace, src/core/MemorySpace.hpp:12 */
+typedef enum bkma_memspace { BKMA_MEM_LOCAL = 0, BKMA_MEM_GLOBAL = 1 } bkma_memspace;
+
+/* Storage type of a device array. */
+typedef enum bkma_dtype {
+    BKMA_F64 = 0,
+    BKMA_F32 = 1,
+    BKMA_BF16 = 2
+} bkma_dtype;
+
+/* ADVParams, src/config/AdvectionParams.hpp:10-57 (POD device copy).
+ * Derived fields (dx, dvx, inv_dx, realWidthX) follow update_deltas,
+ * src/config/AdvectionParams.cpp:96-103; see bkma_adv_update_deltas. */
+typedef struct bkma_adv_params {
+    uint64_t n0, n1, n2;
+    uint64_t maxIter;
+    uint64_t pref_wg_size;
+    uint64_t seq_size0, seq_size2;
+    double dt, dx, dvx, inv_dx;
+    double minRealX, maxRealX, realWidthX;
+    double minRealVx, maxRealVx;
+    float percent_loc;
+    int32_t gpu;
+    int32_t outputSolution;
+    int32_t _pad;
+} bkma_adv_params;
+
+/* BatchConfig1D, src/core/bkma_tools.hpp:113-122 */
+typedef struct bkma_batch_config {
+    uint64_t n_batch, batch_size, last_batch_size;
+} bkma_batch_config;
+
+/* WorkGroupDispatch, src/core/bkma_tools.hpp:25-56 */
+typedef struct bkma_wg_dispatch {
+    uint64_t s0, s2; /* sequential sizes */
+    uint64_t g0, g2; /* work-groups per batch in dims 0 and 2 */
+} bkma_wg_dispatch;
+
+/* BkmaOptimParams, src/core/bkma_tools.hpp:126-134 */
+typedef struct bkma_optim_params {
+    bkma_batch_config dispatch_d0;
+    bkma_batch_config dispatch_d2;
+    uint64_t w0, w1, w2;
+    bkma_wg_dispatch wg_dispatch;
+    int32_t mem_space; /* bkma_memspace */
+    int32_t _pad;
+} bkma_optim_params;
+
+/* A caller-provided global scratch, the span3d_t `global_scratch` argument of
+ * bkma_run (src/core/bkma_run.hpp:10-13).  data == NULL means "empty span". */
+typedef struct bkma_span3d {
+    void *data;
+    uint64_t e0, e1, e2;
+} bkma_span3d;
+
+/* ------------------------------------------------------------------ */
+/* Library / device                                                     */
+/* ------------------------------------------------------------------ */
+int bkma_abi_version(void);
+const char *bkma_last_error(void);
+/* Maximum LDS bytes one work-group may allocate on `device` (the value the
+ * reference reads from sycl::info::device::local_mem_size, init.hpp:84-86). */
+bkma_status bkma_query_lds_bytes(int device, uint64_t *bytes);
+
+/* ------------------------------------------------------------------ */
+/* Parameters and dispatch policy (host arithmetic, no device work)     */
+/* ------------------------------------------------------------------ */
+/* ADVParams::update_deltas, src/config/AdvectionParams.cpp:96-103 */
+void bkma_adv_update_deltas(bkma_adv_params *p);
+
+/* create_optim_params<Params>, src/tools/init.hpp:76-102, composed of
+ * WorkItemDispatch::set_ideal_sizes / adjust_sizes_mem_limit
+ * (src/core/bkma_tools.hpp:74-107) and WorkGroupDispatch::set_num_work_groups
+ * (src/core/bkma_tools.hpp:34-55).  `local_mem_bytes` = 0 queries the current
+ * device.  `elem_size` is sizeof the scratch element (8 for the reference's
+ * real_t = double).  Returns BKMA_ERR_INVALID_ARGUMENT with the reference's
+ * messages where set_num_work_groups throws, and where the reference divides
+ * by zero (a line longer than the LDS, bkma_tools.hpp:102-105) it falls back
+ * to MemorySpace::Global with w0=w2=1 instead. */
+bkma_status bkma_create_optim_params(uint64_t n0, uint64_t n1, uint64_t n2, uint64_t pref_wg_size,
+                                     uint64_t seq_size0, uint64_t seq_size2, uint64_t elem_size,
+                                     uint64_t local_mem_bytes, bkma_optim_params *out);
+
+/* init_1d_blocking, src/core/bkma_tools.hpp:138-153 */
+bkma_batch_config bkma_init_1d_blocking(uint64_t n, uint64_t max_batchs);
+
+/* Elements of global scratch that bkma_adv_run / bkma_conv_run need for
+ * (impl, optim) on an [n0][n1][n2] array with the given solver window.
+ * AdaptiveWg/Global: extents (g0*w0, g2*w2, n1-window+1), indexed by global
+ * id (src/core/MemorySpace.hpp:45-70).  BasicRange: (n0, n1, n2). */
+bkma_status bkma_scratch_extents(bkma_impl impl, const bkma_optim_params *optim, uint64_t n0, uint64_t n1,
+                                 uint64_t n2, uint64_t window, uint64_t *e0, uint64_t *e1, uint64_t *e2);
+
+/* ------------------------------------------------------------------ */
+/* bkma_run<AdvectionSolver, Impl>  (src/core/bkma_run.hpp:10-62)      */
+/* ------------------------------------------------------------------ */
+/* One in-place advection step of `data` ([n0][n1][n2], dtype F64 or F32).
+ * The solver is AdvectionSolver (src/solvers/AdvectionSolver.hpp:13-99):
+ * FP64 foot location, lag_basis and stencil sum in the reference's exact
+ * operation order; F32 storage is converted on load and rounded on store.
+ * impl/optim/scratch have the reference's meaning: AdaptiveWg uses
+ * optim->mem_space (Local: LDS scratch; Global: `scratch`); NDRange needs
+ * n1 <= 1024; BasicRange needs a (n0, n1, n2) global scratch. */
+bkma_status bkma_adv_run(bkma_impl impl, bkma_dtype dtype, void *data, const bkma_adv_params *params,
+                         const bkma_optim_params *optim, bkma_span3d scratch, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* bkma_run<ConvSolver, Impl>  (src/core/bkma_run.hpp:10-62)           */
+/* ------------------------------------------------------------------ */
+/* ConvSolver (src/solvers/ConvSolver.hpp:7-45) through the BKMA executor,
+ * window() = K: data is [n0][n1 = C*L][n2]; weight is the (K, C_in, C_out)
+ * span of conv1d.cpp:46 with C_in = C_out = C; bias is [C].  Same dtype for
+ * data, weight and bias (F64 or F32).  Writes that the reference performs at
+ * negative window indices (outside the line's scratch) are not performed. */
+bkma_status bkma_conv_run(bkma_impl impl, bkma_dtype dtype, void *data, uint64_t n0, uint64_t n1, uint64_t n2,
+                          const void *weight, const void *bias, uint64_t K, uint64_t C, uint64_t L,
+                          const bkma_optim_params *optim, bkma_span3d scratch, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Harness kernels: fill_buffer_* (src/tools/init.hpp:31-72) and the   */
+/* validators (src/tools/validation.hpp)                                */
+/* ------------------------------------------------------------------ */
+/* U[0,1) from the counter-based generator keyed by (seed, linear index);
+ * bit-identical to the oracle's generator (53 random bits, then rounded to
+ * the storage type). */
+bkma_status bkma_fill_uniform(bkma_dtype dtype, void *out, uint64_t n, uint64_t seed, void *stream);
+/* N(0, stddev^2) by Box-Muller over two generator draws per element. */
+bkma_status bkma_fill_normal(bkma_dtype dtype, void *out, uint64_t n, uint64_t seed, double stddev, void *stream);
+/* fill_buffer_adv, src/tools/init.hpp:31-46: sin(4*x*pi). */
+bkma_status bkma_fill_adv_sin4(bkma_dtype dtype, void *data, const bkma_adv_params *params, void *stream);
+/* validate_result_adv, src/tools/validation.hpp:9-80 (synchronous):
+ * highest / lowest over i2 of mean_(i0,i1) |f - sin(4 pi (x - v t))|,
+ * t = params->maxIter * dt. */
+bkma_status bkma_validate_adv_l1(bkma_dtype dtype, const void *data, const bkma_adv_params *params,
+                                 double *highest, double *lowest, void *stream);
+/* sum_and_normalize_conv1d, src/tools/validation.hpp:84-110 (synchronous). */
+bkma_status bkma_conv_sum_normalize(bkma_dtype dtype, const void *data, uint64_t n0, uint64_t n1, uint64_t n2,
+                                    uint64_t nw, double *mean, void *stream);
+/* validate_conv1d, src/tools/validation.hpp:114-156 (synchronous):
+ * *all_equal = 1 when data(:, i1, :) is constant for every i1 < nw. */
+bkma_status bkma_conv_validate(bkma_dtype dtype, const void *data, uint64_t n0, uint64_t n1, uint64_t n2,
+                               uint64_t nw, int32_t *all_equal, void *stream);
+/* Error norms of `a` against `ref` over n elements (synchronous):
+ * out[0] = max|a-ref|, out[1] = max|ref|, out[2] = sum (a-ref)^2,
+ * out[3] = sum ref^2, all accumulated in FP64. */
+bkma_status bkma_error_norms(bkma_dtype a_dtype, const void *a, bkma_dtype ref_dtype, const void *ref, uint64_t n,
+                             double out[4], void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BKMA_AMD_H */
