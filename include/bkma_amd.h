@@ -165,6 +165,48 @@ bkma_status bkma_conv_run(bkma_impl impl, bkma_dtype dtype, void *data, uint64_t
                           const bkma_optim_params *optim, bkma_span3d scratch, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Tuned advection plans                                               */
+/* ------------------------------------------------------------------ */
+/* Replaces the per-step `bkma_run_function(Q, data, solver, optim_params,
+ * span3d_t{})` call of the advection time loop (src/advection.cpp:51-58)
+ * for callers that want the tuned kernels: a plan is built once per
+ * (ADVParams, storage dtype), chooses a strategy and its launch shape
+ * (hand-set in the reference through BkmaOptimParams / AdaptiveWg), then
+ * runs any number of in-place steps.
+ *  - F64 storage: results bit-identical to bkma_adv_run (the reference's
+ *    FP64 arithmetic, reordered only across cells).
+ *  - F32 storage: FP64 foot location / index / lag_basis exactly as the
+ *    reference, FP64 fused accumulation, one rounding to FP32 per step. */
+typedef enum bkma_strategy {
+    BKMA_STRATEGY_COLUMN_STREAM = 1 /* lines strided by n2: register ring per wavefront */
+} bkma_strategy;
+
+typedef struct bkma_tuned_config {
+    int32_t strategy;     /* bkma_strategy */
+    int32_t vec;          /* consecutive i2 cells per lane */
+    int32_t smax;         /* largest |v*dt/dx| the stencil window covers */
+    int32_t prefetch;     /* rows loaded ahead of the stencil window */
+    int32_t waves_per_wg; /* wavefronts per work-group */
+    int32_t _reserved[3];
+} bkma_tuned_config;
+
+typedef struct bkma_adv_plan bkma_adv_plan;
+
+/* autotune = 0: heuristic choice; 1: time the feasible variants on a scratch
+ * array of the same line shape (allocated and freed inside the call). */
+bkma_status bkma_adv_plan_create(const bkma_adv_params *params, bkma_dtype dtype, int32_t autotune, void *stream,
+                                 bkma_adv_plan **out);
+/* A plan with an explicit configuration (BKMA_ERR_UNSUPPORTED if it is not
+ * compiled in or cannot cover the shape). */
+bkma_status bkma_adv_plan_create_with(const bkma_adv_params *params, bkma_dtype dtype,
+                                      const bkma_tuned_config *cfg, bkma_adv_plan **out);
+bkma_status bkma_adv_plan_describe(const bkma_adv_plan *plan, bkma_tuned_config *cfg, char *name,
+                                   uint64_t name_len);
+/* nsteps in-place steps on `data` (16-byte aligned, the plan's dtype). */
+bkma_status bkma_adv_plan_run(bkma_adv_plan *plan, void *data, uint64_t nsteps, void *stream);
+void bkma_adv_plan_destroy(bkma_adv_plan *plan);
+
+/* ------------------------------------------------------------------ */
 /* Harness kernels: fill_buffer_* (src/tools/init.hpp:31-72) and the   */
 /* validators (src/tools/validation.hpp)                                */
 /* ------------------------------------------------------------------ */

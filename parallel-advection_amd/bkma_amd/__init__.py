@@ -92,6 +92,17 @@ class OptimParams(ctypes.Structure):
                 ("wg_dispatch", WgDispatch), ("mem_space", ctypes.c_int32), ("_pad", ctypes.c_int32)]
 
 
+class TunedConfig(ctypes.Structure):
+    """bkma_tuned_config."""
+
+    _fields_ = [("strategy", ctypes.c_int32), ("vec", ctypes.c_int32), ("smax", ctypes.c_int32),
+                ("prefetch", ctypes.c_int32), ("waves_per_wg", ctypes.c_int32),
+                ("_reserved", ctypes.c_int32 * 3)]
+
+
+STRATEGY_COLUMN_STREAM = 1
+
+
 class Span3d(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("e0", ctypes.c_uint64), ("e1", ctypes.c_uint64),
                 ("e2", ctypes.c_uint64)]
@@ -107,6 +118,8 @@ EXPORTS = [
     "bkma_create_optim_params", "bkma_init_1d_blocking", "bkma_scratch_extents", "bkma_adv_run",
     "bkma_conv_run", "bkma_fill_uniform", "bkma_fill_normal", "bkma_fill_adv_sin4",
     "bkma_validate_adv_l1", "bkma_conv_sum_normalize", "bkma_conv_validate", "bkma_error_norms",
+    "bkma_adv_plan_create", "bkma_adv_plan_create_with", "bkma_adv_plan_describe", "bkma_adv_plan_run",
+    "bkma_adv_plan_destroy",
 ]
 
 
@@ -139,6 +152,11 @@ def lib():
         "bkma_conv_sum_normalize": (st, [ctypes.c_int, vp, u64, u64, u64, u64, P(dbl), vp]),
         "bkma_conv_validate": (st, [ctypes.c_int, vp, u64, u64, u64, u64, P(i32), vp]),
         "bkma_error_norms": (st, [ctypes.c_int, vp, ctypes.c_int, vp, u64, P(dbl), vp]),
+        "bkma_adv_plan_create": (st, [P(AdvParams), ctypes.c_int, i32, vp, P(vp)]),
+        "bkma_adv_plan_create_with": (st, [P(AdvParams), ctypes.c_int, P(TunedConfig), P(vp)]),
+        "bkma_adv_plan_describe": (st, [vp, P(TunedConfig), ctypes.c_char_p, u64]),
+        "bkma_adv_plan_run": (st, [vp, vp, u64, vp]),
+        "bkma_adv_plan_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -278,3 +296,37 @@ def conv_validate(t, nw, stream=None) -> bool:
     f = ctypes.c_int32()
     check(lib().bkma_conv_validate(dtype_code(t), _ptr(t), n0, n1, n2, nw, ctypes.byref(f), _stream(stream)))
     return bool(f.value)
+
+
+class AdvPlan:
+    """A tuned in-place advection plan (bkma_adv_plan_*)."""
+
+    def __init__(self, params, dtype, autotune: bool = False, config: TunedConfig | None = None, stream=None):
+        self._h = ctypes.c_void_p()
+        self.params = params
+        if config is None:
+            check(lib().bkma_adv_plan_create(ctypes.byref(params), dtype, int(autotune), _stream(stream),
+                                             ctypes.byref(self._h)))
+        else:
+            check(lib().bkma_adv_plan_create_with(ctypes.byref(params), dtype, ctypes.byref(config),
+                                                  ctypes.byref(self._h)))
+
+    def run(self, data, nsteps: int = 1, stream=None) -> None:
+        check(lib().bkma_adv_plan_run(self._h, _ptr(data), nsteps, _stream(stream)))
+
+    def describe(self):
+        cfg = TunedConfig()
+        buf = ctypes.create_string_buffer(128)
+        check(lib().bkma_adv_plan_describe(self._h, ctypes.byref(cfg), buf, 128))
+        return cfg, buf.value.decode()
+
+    def close(self) -> None:
+        if self._h:
+            lib().bkma_adv_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

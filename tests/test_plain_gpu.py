@@ -59,7 +59,7 @@ def test_adaptivewg_local_c0_config_bit_exact(cuda):
     _assert_bit_exact(got, ref)
 
 
-@pytest.mark.parametrize("shape,pref", [((16, 256, 33), 256), ((3, 1000, 70), 128), ((160, 16, 4), 512),
+@pytest.mark.parametrize("shape,pref", [((16, 256, 33), 256), ((3, 1000, 70), 128), ((100, 16, 1), 128),
                                         ((5, 2048, 2), 1024)])
 def test_adaptivewg_local_strided(cuda, shape, pref):
     n0, n1, n2 = shape
@@ -72,6 +72,17 @@ def test_adaptivewg_local_strided(cuda, shape, pref):
     ref = host.copy()
     O.advect_f64(ref, po, 7)
     _assert_bit_exact(got, ref)
+
+
+def test_adaptivewg_oversized_work_group_is_rejected(cuda):
+    # w0 = pref / n1 * n2 = 128 gives a 128*16*4 work-group (bkma_tools.hpp:86):
+    # a launch error in the reference, a clear invalid-argument status here.
+    torch = _torch()
+    p = B.make_params(160, 16, 4)
+    o = B.create_optim_params(160, 16, 4, pref_wg_size=512, elem_size=8)
+    d = torch.zeros((160, 16, 4), dtype=torch.float64, device="cuda")
+    with pytest.raises(B.BkmaInvalidArgument, match="exceeds 1024"):
+        B.adv_run(B.ADAPTIVE_WG, d, p, o)
 
 
 def test_adaptivewg_global_scratch(cuda):
@@ -194,7 +205,7 @@ def test_error_norms(cuda):
     a = torch.tensor([1.0, 2.0, 3.0], dtype=torch.float64, device="cuda")
     r = torch.tensor([1.0, 2.5, 2.0], dtype=torch.float32, device="cuda")
     mx, mr, s2, r2 = B.error_norms(a, r)
-    assert (mx, mr) == (1.0, 3.0) and abs(s2 - 1.25) < 1e-15 and abs(r2 - (1 + 6.25 + 4)) < 1e-12
+    assert (mx, mr) == (1.0, 2.5) and abs(s2 - 1.25) < 1e-15 and abs(r2 - (1 + 6.25 + 4)) < 1e-12
 
 
 # ---------------------------------------------------------------------------

@@ -1,0 +1,138 @@
+"""GPU parity of the tuned advection kernels (bkma_adv_plan_*).
+
+FP64 storage must be bit-identical to the oracle's FP64 restatement of the
+reference; FP32 storage must be bit-identical to the oracle's restatement of
+the tuned FP32 arithmetic (FP64 foot/index/coefficients, fused FP64
+accumulation, one FP32 rounding per step) and within 1e-6 relative max-norm
+of the FP64 result."""
+import numpy as np
+import pytest
+
+import bkma_amd as B
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _run(host, params, dtype, nsteps, config=None, autotune=False):
+    torch = _torch()
+    tdt = torch.float64 if dtype == B.F64 else torch.float32
+    d = torch.from_numpy(host).to(dtype=tdt, device="cuda")
+    plan = B.AdvPlan(params, dtype, autotune=autotune, config=config)
+    cfg, name = plan.describe()
+    print("plan", name, flush=True)
+    plan.run(d, nsteps)
+    torch.cuda.synchronize()
+    plan.close()
+    return d.cpu().numpy()
+
+
+def _cfg(vec, smax, prefetch, wpb=1):
+    return B.TunedConfig(strategy=B.STRATEGY_COLUMN_STREAM, vec=vec, smax=smax, prefetch=prefetch,
+                         waves_per_wg=wpb)
+
+
+SHAPES = [(4, 1024, 512), (3, 333, 64), (5, 200, 192), (2, 96, 96), (7, 1000, 130)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_tuned_f64_bit_exact(cuda, shape):
+    n0, n1, n2 = shape
+    p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    po = O.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    host = O.uniform(n0 * n1 * n2, 1).reshape(shape)
+    got = _run(host, p, B.F64, 9)
+    ref = host.copy()
+    O.advect_f64(ref, po, 9)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_tuned_f32_bit_exact_vs_restatement(cuda, shape):
+    n0, n1, n2 = shape
+    p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    po = O.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    host = O.uniform(n0 * n1 * n2, 2, np.float32).reshape(shape)
+    got = _run(host, p, B.F32, 9)
+    ref = host.copy()
+    O.advect_f32_mixed(ref, po, 9)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+
+
+@pytest.mark.parametrize("vec,smax,prefetch,wpb", [(4, 2, 4, 1), (4, 2, 4, 2), (4, 2, 8, 1), (2, 2, 4, 1),
+                                                   (2, 2, 8, 1), (1, 2, 8, 1), (4, 1, 4, 1), (2, 1, 8, 1),
+                                                   (1, 1, 8, 1), (4, 4, 4, 1), (2, 4, 4, 1), (1, 4, 4, 1)])
+def test_every_f32_variant(cuda, vec, smax, prefetch, wpb):
+    n0, n1, n2 = 3, 300, 256
+    # |v*dt/dx| <= 0.9 * smax so that every variant covers the shift
+    dt = 0.9 * smax / n1
+    p = B.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
+    po = O.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
+    host = O.uniform(n0 * n1 * n2, 3, np.float32).reshape(n0, n1, n2)
+    got = _run(host, p, B.F32, 5, config=_cfg(vec, smax, prefetch, wpb))
+    ref = host.copy()
+    O.advect_f32_mixed(ref, po, 5)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("vec,smax,prefetch", [(2, 2, 4), (2, 2, 8), (1, 2, 8), (2, 1, 4), (1, 1, 8), (2, 4, 4),
+                                               (1, 4, 4)])
+def test_every_f64_variant(cuda, vec, smax, prefetch):
+    n0, n1, n2 = 3, 257, 130 if vec == 1 else 128
+    dt = 0.9 * smax / n1
+    p = B.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
+    po = O.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
+    host = O.uniform(n0 * n1 * n2, 4).reshape(n0, n1, n2)
+    got = _run(host, p, B.F64, 5, config=_cfg(vec, smax, prefetch))
+    ref = host.copy()
+    O.advect_f64(ref, po, 5)
+    assert np.array_equal(got, ref)
+
+
+def test_integer_shift_known_answer(cuda):
+    # v*dt/dx integer (dt = 1/512, n1 = 1024, v = -1 + i0/2): circular shift
+    n0, n1, n2 = 4, 1024, 128
+    p = B.make_params(n0, n1, n2, dt=1 / 512, minRealVx=-1, maxRealVx=1)
+    host = O.uniform(n0 * n1 * n2, 5).reshape(n0, n1, n2)
+    got = _run(host, p, B.F64, 1)
+    for i0 in range(n0):
+        s = int(round((-1 + i0 / 2) * 2))
+        assert np.array_equal(got[i0], np.roll(host[i0], s, axis=0))
+
+
+def test_autotune_picks_a_correct_variant(cuda):
+    n0, n1, n2 = 16, 512, 512
+    p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    po = O.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    host = O.uniform(n0 * n1 * n2, 6, np.float32).reshape(n0, n1, n2)
+    got = _run(host, p, B.F32, 3, autotune=True)
+    ref = host.copy()
+    O.advect_f32_mixed(ref, po, 3)
+    assert np.array_equal(got, ref)
+
+
+def test_f32_error_budget_vs_f64(cuda):
+    # 50 steps of the strided benchmark shape, reduced in n0: FP32 storage
+    # within 1e-6 relative max-norm of FP64 (errors computed on the GPU).
+    torch = _torch()
+    n0, n1, n2 = 64, 1024, 512
+    p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    d64 = torch.empty((n0, n1, n2), dtype=torch.float64, device="cuda")
+    B.fill_uniform(d64, 1)
+    d32 = d64.to(torch.float32)
+    B.AdvPlan(p, B.F64).run(d64, 50)
+    B.AdvPlan(p, B.F32).run(d32, 50)
+    rel_max, rel_l2 = B.rel_errors(d32, d64)
+    print(f"FP32 vs FP64 after 50 steps: rel_max {rel_max:.3e} rel_L2 {rel_l2:.3e}", flush=True)
+    assert rel_max <= 1e-6
+
+
+def test_unsupported_shapes_are_reported(cuda):
+    p = B.make_params(2, 1024, 8, dt=1e-1, minRealVx=-1, maxRealVx=1)  # shift ~100 cells
+    with pytest.raises(B.BkmaUnsupported):
+        B.AdvPlan(p, B.F32)
