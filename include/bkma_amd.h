@@ -187,7 +187,9 @@ typedef struct bkma_tuned_config {
     int32_t smax;         /* largest |v*dt/dx| the stencil window covers */
     int32_t prefetch;     /* rows loaded ahead of the stencil window */
     int32_t waves_per_wg; /* wavefronts per work-group */
-    int32_t _reserved[3];
+    int32_t nontemporal;  /* 1: non-temporal global loads/stores */
+    int32_t persistent;   /* 1: grid capped at the resident work-groups, waves loop over units */
+    int32_t _reserved;
 } bkma_tuned_config;
 
 typedef struct bkma_adv_plan bkma_adv_plan;
@@ -205,6 +207,9 @@ bkma_status bkma_adv_plan_describe(const bkma_adv_plan *plan, bkma_tuned_config 
 /* nsteps in-place steps on `data` (16-byte aligned, the plan's dtype). */
 bkma_status bkma_adv_plan_run(bkma_adv_plan *plan, void *data, uint64_t nsteps, void *stream);
 void bkma_adv_plan_destroy(bkma_adv_plan *plan);
+/* The compiled configurations of `dtype` (the autotuner's search space):
+ * writes up to max_out of them and returns how many exist. */
+int32_t bkma_adv_plan_variants(bkma_dtype dtype, bkma_tuned_config *out, int32_t max_out);
 
 /* ------------------------------------------------------------------ */
 /* Harness kernels: fill_buffer_* (src/tools/init.hpp:31-72) and the   */

@@ -97,7 +97,7 @@ class TunedConfig(ctypes.Structure):
 
     _fields_ = [("strategy", ctypes.c_int32), ("vec", ctypes.c_int32), ("smax", ctypes.c_int32),
                 ("prefetch", ctypes.c_int32), ("waves_per_wg", ctypes.c_int32),
-                ("_reserved", ctypes.c_int32 * 3)]
+                ("nontemporal", ctypes.c_int32), ("persistent", ctypes.c_int32), ("_reserved", ctypes.c_int32)]
 
 
 STRATEGY_COLUMN_STREAM = 1
@@ -119,7 +119,7 @@ EXPORTS = [
     "bkma_conv_run", "bkma_fill_uniform", "bkma_fill_normal", "bkma_fill_adv_sin4",
     "bkma_validate_adv_l1", "bkma_conv_sum_normalize", "bkma_conv_validate", "bkma_error_norms",
     "bkma_adv_plan_create", "bkma_adv_plan_create_with", "bkma_adv_plan_describe", "bkma_adv_plan_run",
-    "bkma_adv_plan_destroy",
+    "bkma_adv_plan_destroy", "bkma_adv_plan_variants",
 ]
 
 
@@ -157,6 +157,7 @@ def lib():
         "bkma_adv_plan_describe": (st, [vp, P(TunedConfig), ctypes.c_char_p, u64]),
         "bkma_adv_plan_run": (st, [vp, vp, u64, vp]),
         "bkma_adv_plan_destroy": (None, [vp]),
+        "bkma_adv_plan_variants": (i32, [ctypes.c_int, P(TunedConfig), i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -296,6 +297,14 @@ def conv_validate(t, nw, stream=None) -> bool:
     f = ctypes.c_int32()
     check(lib().bkma_conv_validate(dtype_code(t), _ptr(t), n0, n1, n2, nw, ctypes.byref(f), _stream(stream)))
     return bool(f.value)
+
+
+def plan_variants(dtype):
+    """The compiled tuned-advection configurations for `dtype`."""
+    n = lib().bkma_adv_plan_variants(dtype, None, 0)
+    arr = (TunedConfig * n)()
+    lib().bkma_adv_plan_variants(dtype, arr, n)
+    return list(arr)
 
 
 class AdvPlan:

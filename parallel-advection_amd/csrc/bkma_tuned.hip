@@ -24,171 +24,10 @@
 #include <string>
 #include <vector>
 
-#include <type_traits>
+#include "adv_stream.hpp"
 #include "common.hpp"
 
 namespace bkma {
-
-template <class T, int VEC> struct VecOf;
-template <> struct VecOf<float, 1> { using type = float; };
-template <> struct VecOf<float, 2> { using type = float2; };
-template <> struct VecOf<float, 4> { using type = float4; };
-template <> struct VecOf<double, 1> { using type = double; };
-template <> struct VecOf<double, 2> { using type = double2; };
-
-template <class T, int VEC> struct VecIO {
-    using V = typename VecOf<T, VEC>::type;
-    __device__ __forceinline__ static void load(const T *p, T (&out)[VEC]) {
-        const V v = *reinterpret_cast<const V *>(p);
-        __builtin_memcpy(out, &v, sizeof(V));
-    }
-    __device__ __forceinline__ static void store(T *p, const T (&in)[VEC]) {
-        V v;
-        __builtin_memcpy(&v, in, sizeof(V));
-        *reinterpret_cast<V *>(p) = v;
-    }
-};
-
-// Interpolation of VEC cells from the 6 ring slots s0..s0+5 (mod R).
-template <class T, int VEC, int R, int S0>
-__device__ __forceinline__ void interp_slots(const T (&win)[R][VEC], const double (&c)[6], T (&out)[VEC]) {
-#pragma unroll
-    for (int v = 0; v < VEC; ++v) {
-        double acc = 0.;
-        if constexpr (sizeof(T) == 8) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) acc = acc + c[k] * (double)win[(S0 + k) % R][v];
-        } else {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) acc = fma(c[k], (double)win[(S0 + k) % R][v], acc);
-        }
-        out[v] = (T)acc;
-    }
-}
-
-// Dispatch on the row's backward shift d = i1 - leftNode in [DLO, DHI]:
-// stencil rows are i1 - d - 2 + k, whose ring slot for loop position J is
-// (J + BACK - d - 2 + k) mod R.
-template <class T, int VEC, int R, int J, int BACK, int D, int DHI>
-__device__ __forceinline__ void interp_switch(int d, const T (&win)[R][VEC], const double (&c)[6], T (&out)[VEC]) {
-    if constexpr (D > DHI) {
-#pragma unroll
-        for (int v = 0; v < VEC; ++v) out[v] = (T)NAN;  // unreachable by construction
-    } else {
-        if (d == D) {
-            constexpr int S0 = ((J + BACK - D - 2) % R + R) % R;
-            interp_slots<T, VEC, R, S0>(win, c, out);
-        } else {
-            interp_switch<T, VEC, R, J, BACK, D + 1, DHI>(d, win, c, out);
-        }
-    }
-}
-
-template <int B, int E, class F> __device__ __forceinline__ void static_for(F &&f) {
-    if constexpr (B < E) {
-        f(std::integral_constant<int, B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
-template <class T, int VEC, int SMAX, int P>
-struct StreamCfg {
-    static constexpr int DLO = -(SMAX + 1), DHI = SMAX + 1;
-    static constexpr int BACK = DHI + 2;  // rows behind the output row
-    static constexpr int AHEAD = 3 - DLO; // rows in front of it
-    static constexpr int WS = BACK + AHEAD + 1;
-    static constexpr int R = WS + P;      // ring rows
-};
-
-struct StreamArgs {
-    AdvDev p;
-    i64 tiles_per_slab;  // column tiles of 64*VEC per i0 slab
-    i64 total_waves;     // n0 * tiles_per_slab
-};
-
-template <class T, int VEC, int SMAX, int P, int WPB>
-__global__ __launch_bounds__(64 * WPB) void adv_column_stream_kernel(T *__restrict__ data, StreamArgs a) {
-    using C = StreamCfg<T, VEC, SMAX, P>;
-    constexpr int R = C::R, BACK = C::BACK, AHEAD = C::AHEAD;
-    __shared__ T head_lds[WPB][AHEAD][64 * VEC];
-
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-    const i64 wave = (i64)blockIdx.x * WPB + wv;
-    if (wave >= a.total_waves) return;  // whole wave exits together
-    const AdvDev &p = a.p;
-    const i64 i0 = wave / a.tiles_per_slab;
-    const i64 tile = wave - i0 * a.tiles_per_slab;
-    const i64 col = (tile * 64 + lane) * VEC;
-    const bool lane_ok = col < p.n2;  // VEC divides n2, so the whole vector is valid
-    const i64 n1 = p.n1, n2 = p.n2;
-    T *slab = data + (i0 * n1) * n2 + (lane_ok ? col : 0);
-
-    T win[R][VEC];
-    auto load_row = [&](i64 row, T (&dst)[VEC]) __attribute__((always_inline)) {
-        if (lane_ok) {
-            VecIO<T, VEC>::load(slab + row * n2, dst);
-        } else {
-#pragma unroll
-            for (int v = 0; v < VEC; ++v) dst[v] = (T)0;
-        }
-    };
-
-    // Prologue: logical rows q in [-BACK, AHEAD + P) -> slot q + BACK.
-    static_for<0, BACK + AHEAD + P>([&](auto sc) __attribute__((always_inline)) {
-        constexpr int S = decltype(sc)::value;
-        constexpr int q = S - BACK;
-        const i64 row = q < 0 ? n1 + q : q;
-        load_row(row, win[S]);
-        if constexpr (q >= 0 && q < AHEAD) {
-#pragma unroll
-            for (int v = 0; v < VEC; ++v) head_lds[wv][q][lane * VEC + v] = win[S][v];
-        }
-    });
-
-    double coef[6];
-    double cached_p = NAN;
-    const int half = (int)(n1 / 2);
-    i64 base_row = 0;
-
-    // One output row at ring position J (compile-time, so every ring slot
-    // below is a constant and the ring lives in registers).
-    auto row_step = [&](auto jc) __attribute__((always_inline)) {
-        constexpr int J = decltype(jc)::value;
-        const i64 q = base_row + J;
-        if (q >= n1) return;
-        // Prefetch logical row q + AHEAD + P into the slot of row
-        // q - BACK - 1, which no later output reads.
-        constexpr int slot = (J + R - 1) % R;
-        const i64 qn = q + AHEAD + P;
-        if (qn < n1) {
-            load_row(qn, win[slot]);
-        } else if (qn < n1 + AHEAD) {
-            const int h = (int)(qn - n1);
-#pragma unroll
-            for (int hh = 0; hh < AHEAD; ++hh)
-                if (h == hh) {
-#pragma unroll
-                    for (int v = 0; v < VEC; ++v) win[slot][v] = head_lds[wv][hh][lane * VEC + v];
-                }
-        }
-        // FP64 foot location of row q (shared by every lane of the wave).
-        double d_prev1;
-        const int leftNode = adv_foot(p, (int)i0, (int)q, d_prev1);
-        if (!(d_prev1 == cached_p)) {
-            adv_lag_basis(d_prev1, coef);
-            cached_p = d_prev1;
-        }
-        int d = (int)q - leftNode;
-        if (d > half) d -= (int)n1;
-        if (d < -half) d += (int)n1;
-        T out[VEC];
-        interp_switch<T, VEC, R, J, BACK, C::DLO, C::DHI>(d, win, coef, out);
-        if (lane_ok) VecIO<T, VEC>::store(slab + q * n2, out);
-    };
-
-    for (; base_row < n1; base_row += R) static_for<0, R>(row_step);
-}
 
 // ---------------------------------------------------------------------------
 // Host side: variant table, plan, autotuner
@@ -196,41 +35,61 @@ __global__ __launch_bounds__(64 * WPB) void adv_column_stream_kernel(T *__restri
 namespace {
 
 using LaunchFn = void (*)(void *data, const StreamArgs &a, unsigned grid, hipStream_t s);
+using OccFn = int (*)();
 
-template <class T, int VEC, int SMAX, int P, int WPB>
+template <class T, int VEC, int SMAX, int P, int WPB, bool NT>
 void launch_stream(void *data, const StreamArgs &a, unsigned grid, hipStream_t s) {
-    hipLaunchKernelGGL((adv_column_stream_kernel<T, VEC, SMAX, P, WPB>), dim3(grid), dim3(64 * WPB), 0, s,
+    hipLaunchKernelGGL((adv_column_stream_kernel<T, VEC, SMAX, P, WPB, NT>), dim3(grid), dim3(64 * WPB), 0, s,
                        reinterpret_cast<T *>(data), a);
+}
+
+// Resident work-groups per CU for a variant (0 if the query fails).
+template <class T, int VEC, int SMAX, int P, int WPB, bool NT> int occupancy_stream() {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, adv_column_stream_kernel<T, VEC, SMAX, P, WPB, NT>,
+                                                     64 * WPB, 0) != hipSuccess)
+        return 0;
+    return blocks;
 }
 
 struct StreamVariant {
     bkma_dtype dtype;
-    int vec, smax, prefetch, wpb;
+    int vec, smax, prefetch, wpb, nt;
     LaunchFn fn;
+    OccFn occ;
 };
 
-#define SV(T, DT, V, S, P, W) StreamVariant{DT, V, S, P, W, &launch_stream<T, V, S, P, W>}
+#define SV(T, DT, V, S, P, W, NT) \
+    StreamVariant{DT, V, S, P, W, NT, &launch_stream<T, V, S, P, W, NT>, &occupancy_stream<T, V, S, P, W, NT>}
+// P is WS/2 or WS (WS = 2*SMAX + 6) so the unrolled loop stays one window long.
 const StreamVariant kStreamVariants[] = {
 #ifdef BKMA_TUNED_FEW
-    SV(float, BKMA_F32, 4, 2, 4, 1),
+    SV(float, BKMA_F32, 4, 2, 10, 1, false),
 #else
-    SV(float, BKMA_F32, 4, 2, 4, 1), SV(float, BKMA_F32, 4, 2, 4, 2), SV(float, BKMA_F32, 4, 2, 8, 1),
-    SV(float, BKMA_F32, 2, 2, 4, 1), SV(float, BKMA_F32, 2, 2, 8, 1), SV(float, BKMA_F32, 1, 2, 8, 1),
-    SV(float, BKMA_F32, 4, 1, 4, 1), SV(float, BKMA_F32, 2, 1, 8, 1), SV(float, BKMA_F32, 1, 1, 8, 1),
-    SV(float, BKMA_F32, 4, 4, 4, 1), SV(float, BKMA_F32, 2, 4, 4, 1), SV(float, BKMA_F32, 1, 4, 4, 1),
-    SV(double, BKMA_F64, 2, 2, 4, 1), SV(double, BKMA_F64, 2, 2, 8, 1), SV(double, BKMA_F64, 1, 2, 8, 1),
-    SV(double, BKMA_F64, 2, 1, 4, 1), SV(double, BKMA_F64, 1, 1, 8, 1),
-    SV(double, BKMA_F64, 2, 4, 4, 1), SV(double, BKMA_F64, 1, 4, 4, 1),
+    SV(float, BKMA_F32, 4, 2, 5, 1, false),   SV(float, BKMA_F32, 4, 2, 10, 1, false),
+    SV(float, BKMA_F32, 4, 2, 10, 1, true),   SV(float, BKMA_F32, 2, 2, 10, 1, false),
+    SV(float, BKMA_F32, 2, 2, 10, 1, true),   SV(float, BKMA_F32, 1, 2, 10, 1, false),
+    SV(float, BKMA_F32, 4, 1, 8, 1, false),   SV(float, BKMA_F32, 2, 1, 8, 1, false),
+    SV(float, BKMA_F32, 4, 4, 7, 1, false),   SV(float, BKMA_F32, 2, 4, 14, 1, false),
+    SV(float, BKMA_F32, 1, 4, 14, 1, false),
+    SV(double, BKMA_F64, 2, 2, 5, 1, false),  SV(double, BKMA_F64, 2, 2, 10, 1, false),
+    SV(double, BKMA_F64, 2, 2, 10, 1, true),  SV(double, BKMA_F64, 1, 2, 10, 1, false),
+    SV(double, BKMA_F64, 2, 1, 8, 1, false),  SV(double, BKMA_F64, 2, 4, 7, 1, false),
+    SV(double, BKMA_F64, 1, 4, 14, 1, false),
 #endif
 };
 #undef SV
 
-// Largest |v * dt / dx| over the velocity grid, with the reference's
-// coordinate formulas; the kernel's shift window must cover ceil of it.
-double max_abs_shift(const bkma_adv_params &p) {
+// Range of the backward shift d = i1 - leftNode over the velocity grid.
+// leftNode = floor(i1 - s + e) with s = v*dt/dx and |e| << 1 (rounding), so
+// d lies in [floor(s_min), floor(s_max) + 1]; a 1e-6 margin absorbs e.
+void shift_range(const bkma_adv_params &p, int *dlo, int *dhi) {
     const double v0 = p.minRealVx;
     const double v1 = p.minRealVx + (double)(p.n0 - 1) * p.dvx;
-    return std::max(std::fabs(v0), std::fabs(v1)) * std::fabs(p.dt) * p.inv_dx;
+    const double s0 = v0 * p.dt * p.inv_dx, s1 = v1 * p.dt * p.inv_dx;
+    const double smin = std::min(s0, s1), smax = std::max(s0, s1);
+    *dlo = (int)std::floor(smin - 1e-6);
+    *dhi = (int)std::floor(smax + 1e-6) + 1;
 }
 
 }  // namespace
@@ -240,6 +99,7 @@ struct TunedPlan {
     bkma_dtype dtype;
     bkma_tuned_config cfg;
     const StreamVariant *variant = nullptr;
+    i64 persistent_blocks = 0;  // 0: one wave per unit; else grid cap (resident work-groups)
     std::string name;
 };
 
@@ -250,13 +110,22 @@ static bkma_status stream_feasible(const bkma_adv_params &p, bkma_dtype dt, cons
         if (why) *why = "n2 not a multiple of VEC";
         return BKMA_ERR_UNSUPPORTED;
     }
-    const double s = max_abs_shift(p);
-    if (!(s <= (double)v.smax)) {
+    if (!std::isfinite(p.dt) || !std::isfinite(p.minRealVx) || !std::isfinite(p.dvx) || !(p.realWidthX > 0)) {
+        if (why) *why = "non-finite parameters";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    int dlo = 0, dhi = 0;
+    if (std::fabs(p.dt * p.inv_dx * std::max(std::fabs(p.minRealVx), std::fabs(p.maxRealVx))) > 1e6) {
         if (why) *why = "shift larger than the register window";
         return BKMA_ERR_UNSUPPORTED;
     }
-    const int R = (2 * (v.smax + 1) + 6) + v.prefetch;
-    if (p.n1 < (u64)(2 * R + 8)) {
+    shift_range(p, &dlo, &dhi);
+    if (dlo < -v.smax || dhi > v.smax) {
+        if (why) *why = "shift larger than the register window";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    const int WS = 2 * v.smax + 6;
+    if (p.n1 < (u64)(2 * (WS + v.prefetch) + 8)) {
         if (why) *why = "line shorter than the register ring";
         return BKMA_ERR_UNSUPPORTED;
     }
@@ -272,25 +141,38 @@ static bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s) {
     StreamArgs a;
     a.p = make_adv_dev(pl.params);
     a.tiles_per_slab = (i64)ceil_div(pl.params.n2, (u64)(64 * v.vec));
-    a.total_waves = (i64)pl.params.n0 * a.tiles_per_slab;
-    const u64 grid = ceil_div((u64)a.total_waves, (u64)v.wpb);
-    if (grid == 0 || grid > 0x7fffffffULL) return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: grid out of range");
+    a.total_units = (i64)pl.params.n0 * a.tiles_per_slab;
+    u64 grid = ceil_div((u64)a.total_units, (u64)v.wpb);
+    if (pl.persistent_blocks > 0) grid = std::min<u64>(grid, (u64)pl.persistent_blocks);
+    grid = std::min<u64>(grid, 0x7fffffffULL / (u64)(64 * v.wpb));
+    if (grid == 0) return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: empty grid");
+    a.wave_stride = (i64)grid * v.wpb;
     v.fn(data, a, (unsigned)grid, s);
     return check_launch("adv_column_stream_kernel");
 }
 
-static std::string variant_name(const StreamVariant &v) {
+static std::string variant_name(const StreamVariant &v, bool persistent) {
     return std::string("column_stream<") + (v.dtype == BKMA_F64 ? "f64" : "f32") + ",vec" + std::to_string(v.vec) +
-           ",smax" + std::to_string(v.smax) + ",p" + std::to_string(v.prefetch) + ",wpb" + std::to_string(v.wpb) + ">";
+           ",smax" + std::to_string(v.smax) + ",p" + std::to_string(v.prefetch) + ",wpb" + std::to_string(v.wpb) +
+           (v.nt ? ",nt" : "") + (persistent ? ",persistent" : "") + ">";
+}
+
+// Resident work-groups on the whole device for a variant (persistent grid).
+static i64 resident_blocks(const StreamVariant &v) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return (i64)v.occ() * cus;
 }
 
 // Time one variant on a scratch copy of (up to) `sub_n0` slabs.
-static bkma_status time_variant(const bkma_adv_params &p, const StreamVariant &v, void *buf, int reps,
-                                hipStream_t s, float *ms) {
+static bkma_status time_variant(const bkma_adv_params &p, const StreamVariant &v, i64 persistent_blocks, void *buf,
+                                int reps, hipStream_t s, float *ms) {
     TunedPlan pl;
     pl.params = p;
     pl.dtype = v.dtype;
     pl.variant = &v;
+    pl.persistent_blocks = persistent_blocks;
     hipEvent_t e0, e1;
     BKMA_HIP_TRY(hipEventCreate(&e0));
     BKMA_HIP_TRY(hipEventCreate(&e1));
@@ -314,6 +196,9 @@ using namespace bkma;
 
 struct bkma_adv_plan : TunedPlan {};
 
+static bkma_adv_plan *make_plan(const bkma_adv_params &p, bkma_dtype dtype, const StreamVariant &v,
+                                i64 persistent_blocks);
+
 extern "C" bkma_status bkma_adv_plan_create(const bkma_adv_params *params, bkma_dtype dtype, int32_t autotune,
                                             void *stream, bkma_adv_plan **out) {
     if (!params || !out) return fail(BKMA_ERR_INVALID_ARGUMENT, "null argument");
@@ -325,18 +210,26 @@ extern "C" bkma_status bkma_adv_plan_create(const bkma_adv_params *params, bkma_
 
     std::vector<const StreamVariant *> cands;
     std::string why;
+    int smin_needed = 1 << 30;
     for (const auto &v : kStreamVariants)
-        if (stream_feasible(*params, dtype, v, &why) == BKMA_OK) cands.push_back(&v);
+        if (stream_feasible(*params, dtype, v, &why) == BKMA_OK) {
+            cands.push_back(&v);
+            smin_needed = std::min(smin_needed, v.smax);
+        }
     if (cands.empty())
         return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: no strategy covers this shape (" + why + ")");
 
-    // Heuristic default: widest vector, smallest window, P = 4, 1 wave/WG.
-    auto score = [](const StreamVariant *v) { return v->vec * 1000 - v->smax * 10 - (v->prefetch == 4 ? 0 : 1) - v->wpb; };
+    // Heuristic default: smallest window that covers the shift, widest vector,
+    // deepest prefetch, temporal stores, one wave per unit.
+    auto score = [](const StreamVariant *v) {
+        return -v->smax * 100000 + v->vec * 1000 + v->prefetch * 10 - v->nt - v->wpb;
+    };
     const StreamVariant *best = *std::max_element(cands.begin(), cands.end(),
                                                   [&](auto *a, auto *b) { return score(a) < score(b); });
+    i64 best_persist = 0;
 
     if (autotune && cands.size() > 1) {
-        // Time candidates on a scratch array holding a sub-problem of whole
+        // Time candidates (both grid modes) on a scratch array of whole
         // slabs large enough to fill the GPU several times over.
         bkma_adv_params sub = *params;
         const u64 elems_per_slab = params->n1 * params->n2;
@@ -348,28 +241,23 @@ extern "C" bkma_status bkma_adv_plan_create(const bkma_adv_params *params, bkma_
         (void)hipMemsetAsync(buf, 0, bytes, as_stream(stream));
         float best_ms = 1e30f;
         for (auto *v : cands) {
-            if (v->smax > 2 * std::max(1, (int)std::ceil(max_abs_shift(*params))) && v->smax > 1) continue;
-            float ms = 0;
-            if (time_variant(sub, *v, buf, 3, as_stream(stream), &ms) == BKMA_OK && ms < best_ms) {
-                best_ms = ms;
-                best = v;
+            if (v->smax > smin_needed + 2) continue;
+            for (int mode = 0; mode < 2; ++mode) {
+                const i64 pb = mode ? resident_blocks(*v) : 0;
+                if (mode && pb <= 0) continue;
+                float ms = 0;
+                if (time_variant(sub, *v, pb, buf, 3, as_stream(stream), &ms) == BKMA_OK && ms < best_ms) {
+                    best_ms = ms;
+                    best = v;
+                    best_persist = pb;
+                }
             }
         }
         (void)hipFree(buf);
+        (void)hipGetLastError();
     }
-
-    auto *pl = new (std::nothrow) bkma_adv_plan();
+    bkma_adv_plan *pl = make_plan(*params, dtype, *best, best_persist);
     if (!pl) return fail(BKMA_ERR_OUT_OF_MEMORY, "plan allocation");
-    pl->params = *params;
-    pl->dtype = dtype;
-    pl->variant = best;
-    std::memset(&pl->cfg, 0, sizeof(pl->cfg));
-    pl->cfg.strategy = BKMA_STRATEGY_COLUMN_STREAM;
-    pl->cfg.vec = best->vec;
-    pl->cfg.smax = best->smax;
-    pl->cfg.prefetch = best->prefetch;
-    pl->cfg.waves_per_wg = best->wpb;
-    pl->name = variant_name(*best);
     *out = pl;
     return BKMA_OK;
 }
@@ -383,22 +271,38 @@ extern "C" bkma_status bkma_adv_plan_create_with(const bkma_adv_params *params, 
         return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: unknown strategy");
     for (const auto &v : kStreamVariants) {
         if (v.dtype == dtype && v.vec == cfg->vec && v.smax == cfg->smax && v.prefetch == cfg->prefetch &&
-            v.wpb == cfg->waves_per_wg) {
+            v.wpb == cfg->waves_per_wg && v.nt == (cfg->nontemporal != 0)) {
             std::string why;
             if (stream_feasible(*params, dtype, v, &why) != BKMA_OK)
                 return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
-            auto *pl = new (std::nothrow) bkma_adv_plan();
+            const i64 pb = cfg->persistent ? resident_blocks(v) : 0;
+            bkma_adv_plan *pl = make_plan(*params, dtype, v, pb);
             if (!pl) return fail(BKMA_ERR_OUT_OF_MEMORY, "plan allocation");
-            pl->params = *params;
-            pl->dtype = dtype;
-            pl->variant = &v;
-            pl->cfg = *cfg;
-            pl->name = variant_name(v);
             *out = pl;
             return BKMA_OK;
         }
     }
     return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: no compiled variant matches the configuration");
+}
+
+static bkma_adv_plan *make_plan(const bkma_adv_params &p, bkma_dtype dtype, const StreamVariant &v,
+                                i64 persistent_blocks) {
+    auto *pl = new (std::nothrow) bkma_adv_plan();
+    if (!pl) return nullptr;
+    pl->params = p;
+    pl->dtype = dtype;
+    pl->variant = &v;
+    pl->persistent_blocks = persistent_blocks;
+    std::memset(&pl->cfg, 0, sizeof(pl->cfg));
+    pl->cfg.strategy = BKMA_STRATEGY_COLUMN_STREAM;
+    pl->cfg.vec = v.vec;
+    pl->cfg.smax = v.smax;
+    pl->cfg.prefetch = v.prefetch;
+    pl->cfg.waves_per_wg = v.wpb;
+    pl->cfg.nontemporal = v.nt;
+    pl->cfg.persistent = persistent_blocks > 0;
+    pl->name = variant_name(v, persistent_blocks > 0);
+    return pl;
 }
 
 extern "C" bkma_status bkma_adv_plan_describe(const bkma_adv_plan *plan, bkma_tuned_config *cfg, char *name,
@@ -421,3 +325,23 @@ extern "C" bkma_status bkma_adv_plan_run(bkma_adv_plan *plan, void *data, uint64
 }
 
 extern "C" void bkma_adv_plan_destroy(bkma_adv_plan *plan) { delete plan; }
+
+extern "C" int32_t bkma_adv_plan_variants(bkma_dtype dtype, bkma_tuned_config *out, int32_t max_out) {
+    int32_t n = 0;
+    for (const auto &v : kStreamVariants) {
+        if (v.dtype != dtype) continue;
+        if (out && n < max_out) {
+            bkma_tuned_config c;
+            std::memset(&c, 0, sizeof(c));
+            c.strategy = BKMA_STRATEGY_COLUMN_STREAM;
+            c.vec = v.vec;
+            c.smax = v.smax;
+            c.prefetch = v.prefetch;
+            c.waves_per_wg = v.wpb;
+            c.nontemporal = v.nt;
+            out[n] = c;
+        }
+        ++n;
+    }
+    return n;
+}

@@ -225,8 +225,11 @@ def test_conv_literal_constant_known_answer(cuda):
     nw = L - K + 1
     assert B.conv_validate(data, nw)
     host = data.cpu().numpy()
-    for oc in range(C):
-        assert np.all(host[:, oc * L: oc * L + nw, :] == 1 + C * K * 1.5 * 7.3)
+    for oc in range(C):  # 1 + C*K*1.5*7.3 = 99.55 up to the order of the 9 additions
+        np.testing.assert_allclose(host[:, oc * L: oc * L + nw, :], 1 + C * K * 1.5 * 7.3, rtol=1e-14)
+    ref = np.full((n0, n1, n2), 7.3)
+    O.conv_literal_f64(ref, np.full((K, C, C), 1.5), np.ones(C), K, C, L)
+    assert np.array_equal(host, ref)
 
 
 @pytest.mark.parametrize("n0,n2,C,L,K", [(3, 2, 4, 37, 5), (2, 5, 8, 64, 3), (4, 1, 3, 1024, 3)])

@@ -32,11 +32,6 @@ def _run(host, params, dtype, nsteps, config=None, autotune=False):
     return d.cpu().numpy()
 
 
-def _cfg(vec, smax, prefetch, wpb=1):
-    return B.TunedConfig(strategy=B.STRATEGY_COLUMN_STREAM, vec=vec, smax=smax, prefetch=prefetch,
-                         waves_per_wg=wpb)
-
-
 SHAPES = [(4, 1024, 512), (3, 333, 64), (5, 200, 192), (2, 96, 96), (7, 1000, 130)]
 
 
@@ -64,31 +59,38 @@ def test_tuned_f32_bit_exact_vs_restatement(cuda, shape):
     assert np.array_equal(got, ref), np.abs(got - ref).max()
 
 
-@pytest.mark.parametrize("vec,smax,prefetch,wpb", [(4, 2, 4, 1), (4, 2, 4, 2), (4, 2, 8, 1), (2, 2, 4, 1),
-                                                   (2, 2, 8, 1), (1, 2, 8, 1), (4, 1, 4, 1), (2, 1, 8, 1),
-                                                   (1, 1, 8, 1), (4, 4, 4, 1), (2, 4, 4, 1), (1, 4, 4, 1)])
-def test_every_f32_variant(cuda, vec, smax, prefetch, wpb):
+def _variant_id(c):
+    return f"vec{c.vec}-smax{c.smax}-p{c.prefetch}-wpb{c.waves_per_wg}{'-nt' if c.nontemporal else ''}"
+
+
+F32_VARIANTS = B.plan_variants(B.F32)
+F64_VARIANTS = B.plan_variants(B.F64)
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("cfg", F32_VARIANTS, ids=_variant_id)
+def test_every_f32_variant(cuda, cfg, persistent):
     n0, n1, n2 = 3, 300, 256
-    # |v*dt/dx| <= 0.9 * smax so that every variant covers the shift
-    dt = 0.9 * smax / n1
+    # |v*dt/dx| <= 0.9 * (smax - 1) so that every variant covers the shift
+    dt = 0.9 * max(cfg.smax - 1, 0.25) / n1
     p = B.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
     po = O.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
     host = O.uniform(n0 * n1 * n2, 3, np.float32).reshape(n0, n1, n2)
-    got = _run(host, p, B.F32, 5, config=_cfg(vec, smax, prefetch, wpb))
+    cfg.persistent = persistent
+    got = _run(host, p, B.F32, 5, config=cfg)
     ref = host.copy()
     O.advect_f32_mixed(ref, po, 5)
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("vec,smax,prefetch", [(2, 2, 4), (2, 2, 8), (1, 2, 8), (2, 1, 4), (1, 1, 8), (2, 4, 4),
-                                               (1, 4, 4)])
-def test_every_f64_variant(cuda, vec, smax, prefetch):
-    n0, n1, n2 = 3, 257, 130 if vec == 1 else 128
-    dt = 0.9 * smax / n1
+@pytest.mark.parametrize("cfg", F64_VARIANTS, ids=_variant_id)
+def test_every_f64_variant(cuda, cfg):
+    n0, n1, n2 = 3, 257, 130 if cfg.vec == 1 else 128
+    dt = 0.9 * max(cfg.smax - 1, 0.25) / n1
     p = B.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
     po = O.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
     host = O.uniform(n0 * n1 * n2, 4).reshape(n0, n1, n2)
-    got = _run(host, p, B.F64, 5, config=_cfg(vec, smax, prefetch))
+    got = _run(host, p, B.F64, 5, config=cfg)
     ref = host.copy()
     O.advect_f64(ref, po, 5)
     assert np.array_equal(got, ref)
