@@ -178,7 +178,8 @@ bkma_status bkma_conv_run(bkma_impl impl, bkma_dtype dtype, void *data, uint64_t
  *  - F32 storage: FP64 foot location / index / lag_basis exactly as the
  *    reference, FP64 fused accumulation, one rounding to FP32 per step. */
 typedef enum bkma_strategy {
-    BKMA_STRATEGY_COLUMN_STREAM = 1 /* lines strided by n2: register ring per wavefront */
+    BKMA_STRATEGY_COLUMN_STREAM = 1, /* lines strided by n2 (n2 >= 64), small shifts: register ring per wavefront */
+    BKMA_STRATEGY_LINE_LDS = 2       /* any n2 and shift: slab segments + halos staged in LDS */
 } bkma_strategy;
 
 typedef struct bkma_tuned_config {
@@ -189,7 +190,7 @@ typedef struct bkma_tuned_config {
     int32_t waves_per_wg; /* wavefronts per work-group */
     int32_t nontemporal;  /* 1: non-temporal global loads/stores */
     int32_t persistent;   /* 1: grid capped at the resident work-groups, waves loop over units */
-    int32_t _reserved;
+    int32_t seg_rows;     /* LINE_LDS: rows per segment (0 = whole line in LDS) */
 } bkma_tuned_config;
 
 typedef struct bkma_adv_plan bkma_adv_plan;

@@ -1,0 +1,153 @@
+// Device side of the LDS-staged advection kernels (see bkma_tuned.hip).
+//
+// A work unit is one segment [r0, r1) of the rows of one i0 slab (all n2
+// columns).  The segment and the halo rows its stencils reach outside it are
+// staged in LDS, then every output of the segment is computed from LDS and
+// written once to HBM.
+//  - Whole-line mode (one segment per slab): the halo is the line's own wrap
+//    around, read with modular indexing; nothing else is needed.
+//  - Segmented mode (lines longer than the LDS budget): other work-groups
+//    overwrite the rows around a segment concurrently, so the halo rows are
+//    first copied to a scratch buffer by halo_snapshot_kernel; the halo of
+//    slab i0 only spans the rows its own velocity reaches (the stencil of
+//    output i1 reads rows i1-d-2 .. i1-d+3 with d in [floor(s), floor(s)+1],
+//    s = v(i0)*dt/dx).
+#pragma once
+
+#include "common.hpp"
+
+namespace bkma {
+
+struct LdsArgs {
+    AdvDev p;
+    i64 seg_rows;  // rows per segment (== n1 in whole-line mode)
+    i64 nseg;      // segments per slab
+    i64 hb_max;    // halo rows before a segment (allocation bound)
+    i64 ha_max;    // halo rows after a segment (allocation bound)
+};
+
+// Halo extents of slab i0: rows [r0 - hb, r0) and [r1, r1 + ha).
+__device__ __forceinline__ void slab_halo(const AdvDev &p, i64 i0, i64 hb_max, i64 ha_max, int &hb, int &ha) {
+    const double v = adv_coord((int)i0, p.minRealVx, p.dvx);
+    const double s = v * p.dt * p.inv_dx;
+    const double dlo = floor(s - 1e-6), dhi = floor(s + 1e-6) + 1.0;
+    hb = (int)fmin((double)hb_max, fmax(0.0, dhi + 2.0));
+    ha = (int)fmin((double)ha_max, fmax(0.0, 3.0 - dlo));
+}
+
+// Copies, for every (slab, segment), the old halo rows into
+// halo[(i0 * nseg + seg) * (hb_max + ha_max) * n2 ...]: first the hb rows
+// before the segment, then the ha rows after it (periodic in i1).
+template <class T>
+__global__ void halo_snapshot_kernel(const T *__restrict__ data, T *__restrict__ halo, LdsArgs a) {
+    const AdvDev &p = a.p;
+    const i64 unit = blockIdx.x;
+    const i64 i0 = unit / a.nseg, seg = unit - i0 * a.nseg;
+    const i64 n1 = p.n1, n2 = p.n2;
+    const i64 r0 = seg * a.seg_rows;
+    const i64 r1 = min(n1, r0 + a.seg_rows);
+    int hb, ha;
+    slab_halo(p, i0, a.hb_max, a.ha_max, hb, ha);
+    const T *slab = data + i0 * n1 * n2;
+    T *dst = halo + unit * (a.hb_max + a.ha_max) * n2;
+    const i64 total = (i64)(hb + ha) * n2;
+    for (i64 e = threadIdx.x; e < total; e += blockDim.x) {
+        const i64 hr = e / n2, c = e - hr * n2;
+        i64 row = hr < hb ? r0 - hb + hr : r1 + (hr - hb);
+        row %= n1;
+        if (row < 0) row += n1;
+        dst[e] = slab[row * n2 + c];
+    }
+}
+
+template <class T, bool SEG>
+__global__ __launch_bounds__(256) void adv_line_lds_kernel(T *__restrict__ data, const T *__restrict__ halo,
+                                                           LdsArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T *lds = reinterpret_cast<T *>(lds_raw);
+    const AdvDev &p = a.p;
+    const i64 n1 = p.n1, n2 = p.n2;
+    const i64 unit = blockIdx.x;
+    const i64 i0 = unit / a.nseg, seg = unit - i0 * a.nseg;
+    const i64 r0 = seg * a.seg_rows;
+    const i64 r1 = min(n1, r0 + a.seg_rows);
+    const i64 nr = r1 - r0;
+    T *slab = data + i0 * n1 * n2;
+
+    int hb = 0, ha = 0;
+    if constexpr (SEG) slab_halo(p, i0, a.hb_max, a.ha_max, hb, ha);
+    const i64 lds_rows = hb + nr + ha;
+
+    // Stage: the segment (contiguous in HBM), then the halo rows.
+    {
+        const i64 cnt = nr * n2;
+        const T *src = slab + r0 * n2;
+        T *dst = lds + (i64)hb * n2;
+        constexpr int V = 16 / sizeof(T);
+        const bool vec_ok = ((reinterpret_cast<uintptr_t>(src) | (uintptr_t)(hb * n2 * sizeof(T))) % 16) == 0;
+        if (vec_ok) {
+            using VT = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+            const i64 nv = cnt / V;
+            for (i64 e = threadIdx.x; e < nv; e += blockDim.x)
+                reinterpret_cast<VT *>(dst)[e] = reinterpret_cast<const VT *>(src)[e];
+            for (i64 e = nv * V + threadIdx.x; e < cnt; e += blockDim.x) dst[e] = src[e];
+        } else {
+            for (i64 e = threadIdx.x; e < cnt; e += blockDim.x) dst[e] = src[e];
+        }
+        if constexpr (SEG) {
+            const T *hsrc = halo + unit * (a.hb_max + a.ha_max) * n2;
+            for (i64 e = threadIdx.x; e < (i64)hb * n2; e += blockDim.x) lds[e] = hsrc[e];
+            for (i64 e = threadIdx.x; e < (i64)ha * n2; e += blockDim.x)
+                lds[(hb + nr) * n2 + e] = hsrc[(i64)hb * n2 + e];
+        }
+    }
+    __syncthreads();
+
+    // Compute: thread (row offset, column); columns beyond 256 loop.
+    const int cw = (int)min<i64>(n2, 256);
+    const int rows_per_pass = 256 / cw;
+    const int t_row = threadIdx.x / cw, t_col = threadIdx.x - t_row * cw;
+    if (t_row >= rows_per_pass) return;  // no barrier follows
+    const int half = (int)(n1 / 2);
+    double coef[6];
+    unsigned long long cached = ~0ull;
+    for (i64 rr = t_row; rr < nr; rr += rows_per_pass) {
+        const i64 i1 = r0 + rr;
+        double d_prev1;
+        const int leftNode = adv_foot(p, (int)i0, (int)i1, d_prev1);
+        const unsigned long long bits = __builtin_bit_cast(unsigned long long, d_prev1);
+        if (bits != cached) {
+            adv_lag_basis(d_prev1, coef);
+            cached = bits;
+        }
+        const int ipos1 = leftNode - 2;
+        i64 lrow;      // LDS row of the first stencil point
+        bool ok = true;
+        if constexpr (SEG) {
+            int delta = ipos1 - (int)i1;
+            if (delta < -half) delta += (int)n1;
+            if (delta > half) delta -= (int)n1;
+            lrow = hb + rr + delta;
+            ok = lrow >= 0 && lrow + 5 < lds_rows;
+        } else {
+            lrow = stencil_fast_ok(n1, ipos1) ? stencil_base(n1, ipos1) : stencil_index_slow(n1, ipos1, 0);
+        }
+        for (i64 c = t_col; c < n2; c += cw) {
+            double acc = 0.;
+            i64 r = lrow;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const double f = ok ? (double)lds[r * n2 + c] : (double)NAN;
+                if constexpr (sizeof(T) == 8) acc = acc + coef[k] * f;
+                else acc = fma(coef[k], f, acc);
+                r += 1;
+                if constexpr (!SEG) {
+                    if (r >= n1) r -= n1;
+                }
+            }
+            slab[i1 * n2 + c] = (T)acc;
+        }
+    }
+}
+
+}  // namespace bkma

@@ -24,6 +24,7 @@
 #include <string>
 #include <vector>
 
+#include "adv_lds.hpp"
 #include "adv_stream.hpp"
 #include "common.hpp"
 
@@ -94,31 +95,62 @@ void shift_range(const bkma_adv_params &p, int *dlo, int *dhi) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
 struct TunedPlan {
     bkma_adv_params params;
     bkma_dtype dtype;
     bkma_tuned_config cfg;
+    std::string name;
+    // COLUMN_STREAM
     const StreamVariant *variant = nullptr;
     i64 persistent_blocks = 0;  // 0: one wave per unit; else grid cap (resident work-groups)
-    std::string name;
+    // LINE_LDS
+    LdsArgs lds{};
+    u64 lds_bytes = 0;
+    void *halo = nullptr;  // segmented mode: halo snapshot buffer (owned)
+    ~TunedPlan() {
+        if (halo) (void)hipFree(halo);
+    }
 };
 
-static bkma_status stream_feasible(const bkma_adv_params &p, bkma_dtype dt, const StreamVariant &v,
-                                   std::string *why) {
+namespace {
+
+bool params_finite(const bkma_adv_params &p) {
+    return std::isfinite(p.dt) && std::isfinite(p.minRealVx) && std::isfinite(p.dvx) && std::isfinite(p.inv_dx) &&
+           std::isfinite(p.minRealX) && p.realWidthX > 0 && std::isfinite(p.realWidthX);
+}
+
+// Every tuned strategy assumes |v*dt/dx| well below n1/2 (the reference's
+// unsigned stencil index is then the plain modulus).
+bool shift_supported(const bkma_adv_params &p, std::string *why) {
+    if (!params_finite(p)) {
+        if (why) *why = "non-finite parameters";
+        return false;
+    }
+    if (p.n1 > 0x7fffffffULL || p.n0 > 0x7fffffffULL) {
+        if (why) *why = "n0/n1 beyond int range";
+        return false;
+    }
+    int dlo, dhi;
+    const double smax = std::fabs(p.dt * p.inv_dx) * std::max(std::fabs(p.minRealVx), std::fabs(p.maxRealVx));
+    if (!(smax < (double)p.n1 / 2 - 8)) {
+        if (why) *why = "shift of half a line or more";
+        return false;
+    }
+    shift_range(p, &dlo, &dhi);
+    return true;
+}
+
+bkma_status stream_feasible(const bkma_adv_params &p, bkma_dtype dt, const StreamVariant &v, std::string *why) {
     if (v.dtype != dt) return BKMA_ERR_UNSUPPORTED;
+    if (!shift_supported(p, why)) return BKMA_ERR_UNSUPPORTED;
     if (p.n2 % (u64)v.vec != 0) {
         if (why) *why = "n2 not a multiple of VEC";
         return BKMA_ERR_UNSUPPORTED;
     }
-    if (!std::isfinite(p.dt) || !std::isfinite(p.minRealVx) || !std::isfinite(p.dvx) || !(p.realWidthX > 0)) {
-        if (why) *why = "non-finite parameters";
-        return BKMA_ERR_UNSUPPORTED;
-    }
     int dlo = 0, dhi = 0;
-    if (std::fabs(p.dt * p.inv_dx * std::max(std::fabs(p.minRealVx), std::fabs(p.maxRealVx))) > 1e6) {
-        if (why) *why = "shift larger than the register window";
-        return BKMA_ERR_UNSUPPORTED;
-    }
     shift_range(p, &dlo, &dhi);
     if (dlo < -v.smax || dhi > v.smax) {
         if (why) *why = "shift larger than the register window";
@@ -129,14 +161,62 @@ static bkma_status stream_feasible(const bkma_adv_params &p, bkma_dtype dt, cons
         if (why) *why = "line shorter than the register ring";
         return BKMA_ERR_UNSUPPORTED;
     }
-    if (p.n1 > 0x7fffffffULL || p.n0 > 0x7fffffffULL) {
-        if (why) *why = "n0/n1 beyond int range";
+    return BKMA_OK;
+}
+
+// LINE_LDS shape for a given segment length (0 = whole line).
+bkma_status lds_shape(const bkma_adv_params &p, bkma_dtype dt, i64 seg_rows, LdsArgs *a, u64 *bytes,
+                      std::string *why) {
+    if (!shift_supported(p, why)) return BKMA_ERR_UNSUPPORTED;
+    u64 lds_max = 0;
+    BKMA_TRY(lds_limit_bytes(&lds_max));
+    const u64 esz = dt == BKMA_F64 ? 8 : 4;
+    int dlo, dhi;
+    shift_range(p, &dlo, &dhi);
+    a->p = make_adv_dev(p);
+    if (seg_rows <= 0 || (u64)seg_rows >= p.n1) {
+        a->seg_rows = (i64)p.n1;
+        a->nseg = 1;
+        a->hb_max = a->ha_max = 0;
+    } else {
+        a->seg_rows = seg_rows;
+        a->nseg = (i64)ceil_div(p.n1, (u64)seg_rows);
+        a->hb_max = std::max(0, dhi + 2);
+        a->ha_max = std::max(0, 3 - dlo);
+    }
+    *bytes = (u64)(a->seg_rows + a->hb_max + a->ha_max) * p.n2 * esz;
+    if (*bytes > lds_max) {
+        if (why) *why = "segment does not fit in LDS";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    if ((u64)p.n0 * (u64)a->nseg > 0x7fffffffULL) {
+        if (why) *why = "too many work units";
         return BKMA_ERR_UNSUPPORTED;
     }
     return BKMA_OK;
 }
 
-static bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s) {
+template <class T>
+bkma_status launch_lds(const TunedPlan &pl, void *data, hipStream_t s) {
+    const LdsArgs &a = pl.lds;
+    const unsigned units = (unsigned)((i64)pl.params.n0 * a.nseg);
+    if (a.nseg > 1) {
+        hipLaunchKernelGGL(halo_snapshot_kernel<T>, dim3(units), dim3(256), 0, s, (const T *)data, (T *)pl.halo, a);
+        BKMA_TRY(check_launch("halo_snapshot_kernel"));
+        auto k = adv_line_lds_kernel<T, true>;
+        BKMA_HIP_TRY(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds_bytes));
+        hipLaunchKernelGGL(k, dim3(units), dim3(256), (size_t)pl.lds_bytes, s, (T *)data, (const T *)pl.halo, a);
+    } else {
+        auto k = adv_line_lds_kernel<T, false>;
+        BKMA_HIP_TRY(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds_bytes));
+        hipLaunchKernelGGL(k, dim3(units), dim3(256), (size_t)pl.lds_bytes, s, (T *)data, (const T *)nullptr, a);
+    }
+    return check_launch("adv_line_lds_kernel");
+}
+
+bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s) {
+    if (pl.cfg.strategy == BKMA_STRATEGY_LINE_LDS)
+        return pl.dtype == BKMA_F64 ? launch_lds<double>(pl, data, s) : launch_lds<float>(pl, data, s);
     const StreamVariant &v = *pl.variant;
     StreamArgs a;
     a.p = make_adv_dev(pl.params);
@@ -151,28 +231,123 @@ static bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s) {
     return check_launch("adv_column_stream_kernel");
 }
 
-static std::string variant_name(const StreamVariant &v, bool persistent) {
+std::string stream_name(const StreamVariant &v, bool persistent) {
     return std::string("column_stream<") + (v.dtype == BKMA_F64 ? "f64" : "f32") + ",vec" + std::to_string(v.vec) +
            ",smax" + std::to_string(v.smax) + ",p" + std::to_string(v.prefetch) + ",wpb" + std::to_string(v.wpb) +
            (v.nt ? ",nt" : "") + (persistent ? ",persistent" : "") + ">";
 }
 
+std::string lds_name(bkma_dtype dt, const LdsArgs &a) {
+    return std::string("line_lds<") + (dt == BKMA_F64 ? "f64" : "f32") +
+           (a.nseg > 1 ? ",seg" + std::to_string(a.seg_rows) + "x" + std::to_string(a.nseg) : ",whole_line") + ">";
+}
+
 // Resident work-groups on the whole device for a variant (persistent grid).
-static i64 resident_blocks(const StreamVariant &v) {
+i64 resident_blocks(const StreamVariant &v) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
     return (i64)v.occ() * cus;
 }
 
-// Time one variant on a scratch copy of (up to) `sub_n0` slabs.
-static bkma_status time_variant(const bkma_adv_params &p, const StreamVariant &v, i64 persistent_blocks, void *buf,
-                                int reps, hipStream_t s, float *ms) {
-    TunedPlan pl;
-    pl.params = p;
-    pl.dtype = v.dtype;
-    pl.variant = &v;
-    pl.persistent_blocks = persistent_blocks;
+// A candidate plan (not yet owning device memory).
+struct Candidate {
+    bkma_tuned_config cfg;
+};
+
+bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned_config &cfg, TunedPlan *pl) {
+    std::string why;
+    pl->params = p;
+    pl->dtype = dt;
+    pl->cfg = cfg;
+    if (cfg.strategy == BKMA_STRATEGY_COLUMN_STREAM) {
+        for (const auto &v : kStreamVariants) {
+            if (v.dtype == dt && v.vec == cfg.vec && v.smax == cfg.smax && v.prefetch == cfg.prefetch &&
+                v.wpb == cfg.waves_per_wg && v.nt == (cfg.nontemporal != 0)) {
+                if (stream_feasible(p, dt, v, &why) != BKMA_OK)
+                    return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
+                pl->variant = &v;
+                pl->persistent_blocks = cfg.persistent ? resident_blocks(v) : 0;
+                pl->name = stream_name(v, pl->persistent_blocks > 0);
+                return BKMA_OK;
+            }
+        }
+        return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: no compiled variant matches the configuration");
+    }
+    if (cfg.strategy == BKMA_STRATEGY_LINE_LDS) {
+        if (lds_shape(p, dt, cfg.seg_rows, &pl->lds, &pl->lds_bytes, &why) != BKMA_OK)
+            return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
+        if (pl->lds.nseg > 1) {
+            const u64 esz = dt == BKMA_F64 ? 8 : 4;
+            const u64 bytes = p.n0 * (u64)pl->lds.nseg * (u64)(pl->lds.hb_max + pl->lds.ha_max) * p.n2 * esz;
+            if (hipMalloc(&pl->halo, std::max<u64>(bytes, 16)) != hipSuccess) {
+                pl->halo = nullptr;
+                return fail(BKMA_ERR_OUT_OF_MEMORY, "tuned advection: halo buffer allocation failed");
+            }
+        }
+        pl->cfg.seg_rows = pl->lds.nseg > 1 ? (int32_t)pl->lds.seg_rows : 0;
+        pl->name = lds_name(dt, pl->lds);
+        return BKMA_OK;
+    }
+    return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: unknown strategy");
+}
+
+// Every feasible configuration: compiled column-stream variants (both grid
+// modes) and LINE_LDS with whole lines or segments sized for 32/64/160 KB.
+std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype dt, std::string *why) {
+    std::vector<bkma_tuned_config> out;
+    int smin_needed = 1 << 30;
+    for (const auto &v : kStreamVariants)
+        if (stream_feasible(p, dt, v, why) == BKMA_OK) smin_needed = std::min(smin_needed, v.smax);
+    for (const auto &v : kStreamVariants) {
+        if (stream_feasible(p, dt, v, why) != BKMA_OK || v.smax > smin_needed + 2) continue;
+        for (int pers = 0; pers < 2; ++pers) {
+            bkma_tuned_config c;
+            std::memset(&c, 0, sizeof(c));
+            c.strategy = BKMA_STRATEGY_COLUMN_STREAM;
+            c.vec = v.vec;
+            c.smax = v.smax;
+            c.prefetch = v.prefetch;
+            c.waves_per_wg = v.wpb;
+            c.nontemporal = v.nt;
+            c.persistent = pers;
+            out.push_back(c);
+        }
+    }
+    const u64 esz = dt == BKMA_F64 ? 8 : 4;
+    LdsArgs a;
+    u64 bytes;
+    std::vector<i64> segs = {0};
+    int dlo = 0, dhi = 0;
+    if (params_finite(p)) shift_range(p, &dlo, &dhi);
+    const i64 halo_rows = std::max(0, dhi + 2) + std::max(0, 3 - dlo);
+    for (u64 budget : {32768ull, 65536ull, 163840ull}) {
+        const i64 rows = (i64)(budget / std::max<u64>(p.n2 * esz, 1)) - halo_rows;
+        if (rows >= 32 && (u64)rows < p.n1) segs.push_back(rows & ~(i64)3);
+    }
+    for (i64 sr : segs) {
+        if (lds_shape(p, dt, sr, &a, &bytes, why) != BKMA_OK) continue;
+        bkma_tuned_config c;
+        std::memset(&c, 0, sizeof(c));
+        c.strategy = BKMA_STRATEGY_LINE_LDS;
+        c.seg_rows = (int32_t)(a.nseg > 1 ? a.seg_rows : 0);
+        bool dup = false;
+        for (auto &o : out) dup |= o.strategy == c.strategy && o.seg_rows == c.seg_rows;
+        if (!dup) out.push_back(c);
+    }
+    return out;
+}
+
+// Heuristic: column stream (smallest window, widest vector, deepest
+// prefetch) when it applies, else LINE_LDS whole line, else the longest
+// segment.
+int heuristic_score(const bkma_tuned_config &c) {
+    if (c.strategy == BKMA_STRATEGY_COLUMN_STREAM)
+        return 1000000000 - c.smax * 100000 + c.vec * 1000 + c.prefetch * 10 - c.nontemporal - c.persistent;
+    return c.seg_rows == 0 ? 500000000 : 100000000 + std::min(c.seg_rows, 1 << 24) / 1024;
+}
+
+bkma_status time_plan(const TunedPlan &pl, void *buf, int reps, hipStream_t s, float *ms) {
     hipEvent_t e0, e1;
     BKMA_HIP_TRY(hipEventCreate(&e0));
     BKMA_HIP_TRY(hipEventCreate(&e1));
@@ -190,14 +365,13 @@ static bkma_status time_variant(const bkma_adv_params &p, const StreamVariant &v
     return st;
 }
 
+}  // namespace
+
 }  // namespace bkma
 
 using namespace bkma;
 
 struct bkma_adv_plan : TunedPlan {};
-
-static bkma_adv_plan *make_plan(const bkma_adv_params &p, bkma_dtype dtype, const StreamVariant &v,
-                                i64 persistent_blocks);
 
 extern "C" bkma_status bkma_adv_plan_create(const bkma_adv_params *params, bkma_dtype dtype, int32_t autotune,
                                             void *stream, bkma_adv_plan **out) {
@@ -208,29 +382,17 @@ extern "C" bkma_status bkma_adv_plan_create(const bkma_adv_params *params, bkma_
         return fail(BKMA_ERR_INVALID_ARGUMENT, "array extents must be positive");
     BKMA_TRY(require_device());
 
-    std::vector<const StreamVariant *> cands;
     std::string why;
-    int smin_needed = 1 << 30;
-    for (const auto &v : kStreamVariants)
-        if (stream_feasible(*params, dtype, v, &why) == BKMA_OK) {
-            cands.push_back(&v);
-            smin_needed = std::min(smin_needed, v.smax);
-        }
+    std::vector<bkma_tuned_config> cands = candidates(*params, dtype, &why);
     if (cands.empty())
         return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: no strategy covers this shape (" + why + ")");
-
-    // Heuristic default: smallest window that covers the shift, widest vector,
-    // deepest prefetch, temporal stores, one wave per unit.
-    auto score = [](const StreamVariant *v) {
-        return -v->smax * 100000 + v->vec * 1000 + v->prefetch * 10 - v->nt - v->wpb;
-    };
-    const StreamVariant *best = *std::max_element(cands.begin(), cands.end(),
-                                                  [&](auto *a, auto *b) { return score(a) < score(b); });
-    i64 best_persist = 0;
+    bkma_tuned_config best = *std::max_element(cands.begin(), cands.end(), [](const auto &a, const auto &b) {
+        return heuristic_score(a) < heuristic_score(b);
+    });
 
     if (autotune && cands.size() > 1) {
-        // Time candidates (both grid modes) on a scratch array of whole
-        // slabs large enough to fill the GPU several times over.
+        // Time the candidates on a scratch array of whole slabs, large enough
+        // to fill the GPU several times over (contents do not affect timing).
         bkma_adv_params sub = *params;
         const u64 elems_per_slab = params->n1 * params->n2;
         const u64 target = 256ull << 20;  // ~256 Mi cells
@@ -240,24 +402,25 @@ extern "C" bkma_status bkma_adv_plan_create(const bkma_adv_params *params, bkma_
         BKMA_HIP_TRY(hipMalloc(&buf, bytes));
         (void)hipMemsetAsync(buf, 0, bytes, as_stream(stream));
         float best_ms = 1e30f;
-        for (auto *v : cands) {
-            if (v->smax > smin_needed + 2) continue;
-            for (int mode = 0; mode < 2; ++mode) {
-                const i64 pb = mode ? resident_blocks(*v) : 0;
-                if (mode && pb <= 0) continue;
-                float ms = 0;
-                if (time_variant(sub, *v, pb, buf, 3, as_stream(stream), &ms) == BKMA_OK && ms < best_ms) {
-                    best_ms = ms;
-                    best = v;
-                    best_persist = pb;
-                }
+        for (const auto &c : cands) {
+            TunedPlan pl;
+            if (build_plan(sub, dtype, c, &pl) != BKMA_OK) continue;
+            float ms = 0;
+            if (time_plan(pl, buf, 3, as_stream(stream), &ms) == BKMA_OK && ms < best_ms) {
+                best_ms = ms;
+                best = c;
             }
         }
         (void)hipFree(buf);
         (void)hipGetLastError();
     }
-    bkma_adv_plan *pl = make_plan(*params, dtype, *best, best_persist);
+    auto *pl = new (std::nothrow) bkma_adv_plan();
     if (!pl) return fail(BKMA_ERR_OUT_OF_MEMORY, "plan allocation");
+    bkma_status st = build_plan(*params, dtype, best, pl);
+    if (st != BKMA_OK) {
+        delete pl;
+        return st;
+    }
     *out = pl;
     return BKMA_OK;
 }
@@ -266,43 +429,19 @@ extern "C" bkma_status bkma_adv_plan_create_with(const bkma_adv_params *params, 
                                                  const bkma_tuned_config *cfg, bkma_adv_plan **out) {
     if (!params || !cfg || !out) return fail(BKMA_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
+    if (dtype != BKMA_F32 && dtype != BKMA_F64) return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: F32 or F64");
+    if (params->n0 == 0 || params->n1 == 0 || params->n2 == 0)
+        return fail(BKMA_ERR_INVALID_ARGUMENT, "array extents must be positive");
     BKMA_TRY(require_device());
-    if (cfg->strategy != BKMA_STRATEGY_COLUMN_STREAM)
-        return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: unknown strategy");
-    for (const auto &v : kStreamVariants) {
-        if (v.dtype == dtype && v.vec == cfg->vec && v.smax == cfg->smax && v.prefetch == cfg->prefetch &&
-            v.wpb == cfg->waves_per_wg && v.nt == (cfg->nontemporal != 0)) {
-            std::string why;
-            if (stream_feasible(*params, dtype, v, &why) != BKMA_OK)
-                return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
-            const i64 pb = cfg->persistent ? resident_blocks(v) : 0;
-            bkma_adv_plan *pl = make_plan(*params, dtype, v, pb);
-            if (!pl) return fail(BKMA_ERR_OUT_OF_MEMORY, "plan allocation");
-            *out = pl;
-            return BKMA_OK;
-        }
-    }
-    return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: no compiled variant matches the configuration");
-}
-
-static bkma_adv_plan *make_plan(const bkma_adv_params &p, bkma_dtype dtype, const StreamVariant &v,
-                                i64 persistent_blocks) {
     auto *pl = new (std::nothrow) bkma_adv_plan();
-    if (!pl) return nullptr;
-    pl->params = p;
-    pl->dtype = dtype;
-    pl->variant = &v;
-    pl->persistent_blocks = persistent_blocks;
-    std::memset(&pl->cfg, 0, sizeof(pl->cfg));
-    pl->cfg.strategy = BKMA_STRATEGY_COLUMN_STREAM;
-    pl->cfg.vec = v.vec;
-    pl->cfg.smax = v.smax;
-    pl->cfg.prefetch = v.prefetch;
-    pl->cfg.waves_per_wg = v.wpb;
-    pl->cfg.nontemporal = v.nt;
-    pl->cfg.persistent = persistent_blocks > 0;
-    pl->name = variant_name(v, persistent_blocks > 0);
-    return pl;
+    if (!pl) return fail(BKMA_ERR_OUT_OF_MEMORY, "plan allocation");
+    bkma_status st = build_plan(*params, dtype, *cfg, pl);
+    if (st != BKMA_OK) {
+        delete pl;
+        return st;
+    }
+    *out = pl;
+    return BKMA_OK;
 }
 
 extern "C" bkma_status bkma_adv_plan_describe(const bkma_adv_plan *plan, bkma_tuned_config *cfg, char *name,
