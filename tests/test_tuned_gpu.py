@@ -158,7 +158,7 @@ LDS_CASES = [
     (3, 4096, 8, 2e-3, 1000),     # same, segmented (ragged last segment)
     (2, 16384, 4, 1e-3, 1024),    # long line, shift ~16 cells, 16 segments
     (4, 2048, 1, 1e-2, 256),      # shift ~20 cells, segments shorter than the halo span
-    (3, 640, 64, 2e-2, 0),        # n2 = 64 with a shift (~12 cells) beyond the register window
+    (3, 256, 64, 2e-2, 0),        # n2 = 64 with a shift (~5 cells) beyond the register window
 ]
 
 
