@@ -222,6 +222,9 @@ int32_t bkma_adv_plan_variants(bkma_dtype dtype, bkma_tuned_config *out, int32_t
 bkma_status bkma_fill_uniform(bkma_dtype dtype, void *out, uint64_t n, uint64_t seed, void *stream);
 /* N(0, stddev^2) by Box-Muller over two generator draws per element. */
 bkma_status bkma_fill_normal(bkma_dtype dtype, void *out, uint64_t n, uint64_t seed, double stddev, void *stream);
+/* Every element set to `value` (rounded to the storage type); the constant
+ * fills of fill_buffer_conv1d, src/tools/init.hpp:50-72. */
+bkma_status bkma_fill_constant(bkma_dtype dtype, void *out, uint64_t n, double value, void *stream);
 /* fill_buffer_adv, src/tools/init.hpp:31-46: sin(4*x*pi). */
 bkma_status bkma_fill_adv_sin4(bkma_dtype dtype, void *data, const bkma_adv_params *params, void *stream);
 /* validate_result_adv, src/tools/validation.hpp:9-80 (synchronous):

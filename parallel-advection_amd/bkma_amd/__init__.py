@@ -117,7 +117,7 @@ _lib = None
 EXPORTS = [
     "bkma_abi_version", "bkma_last_error", "bkma_query_lds_bytes", "bkma_adv_update_deltas",
     "bkma_create_optim_params", "bkma_init_1d_blocking", "bkma_scratch_extents", "bkma_adv_run",
-    "bkma_conv_run", "bkma_fill_uniform", "bkma_fill_normal", "bkma_fill_adv_sin4",
+    "bkma_conv_run", "bkma_fill_uniform", "bkma_fill_normal", "bkma_fill_constant", "bkma_fill_adv_sin4",
     "bkma_validate_adv_l1", "bkma_conv_sum_normalize", "bkma_conv_validate", "bkma_error_norms",
     "bkma_adv_plan_create", "bkma_adv_plan_create_with", "bkma_adv_plan_describe", "bkma_adv_plan_run",
     "bkma_adv_plan_destroy", "bkma_adv_plan_variants",
@@ -148,6 +148,7 @@ def lib():
                                P(OptimParams), Span3d, vp]),
         "bkma_fill_uniform": (st, [ctypes.c_int, vp, u64, u64, vp]),
         "bkma_fill_normal": (st, [ctypes.c_int, vp, u64, u64, dbl, vp]),
+        "bkma_fill_constant": (st, [ctypes.c_int, vp, u64, dbl, vp]),
         "bkma_fill_adv_sin4": (st, [ctypes.c_int, vp, P(AdvParams), vp]),
         "bkma_validate_adv_l1": (st, [ctypes.c_int, vp, P(AdvParams), P(dbl), P(dbl), vp]),
         "bkma_conv_sum_normalize": (st, [ctypes.c_int, vp, u64, u64, u64, u64, P(dbl), vp]),
@@ -258,6 +259,10 @@ def fill_uniform(t, seed: int, stream=None) -> None:
 
 def fill_normal(t, seed: int, stddev: float = 1.0, stream=None) -> None:
     check(lib().bkma_fill_normal(dtype_code(t), _ptr(t), t.numel(), seed, stddev, _stream(stream)))
+
+
+def fill_constant(t, value: float, stream=None) -> None:
+    check(lib().bkma_fill_constant(dtype_code(t), _ptr(t), t.numel(), value, _stream(stream)))
 
 
 def fill_adv_sin4(t, params, stream=None) -> None:

@@ -92,6 +92,16 @@ template <> __global__ void fill_normal_kernel<__hip_bfloat16>(__hip_bfloat16 *o
     }
 }
 
+template <class T>
+__global__ void fill_constant_kernel(T *out, i64 n, double value) {
+    const T v = (T)value;
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) out[i] = v;
+}
+template <> __global__ void fill_constant_kernel<__hip_bfloat16>(__hip_bfloat16 *out, i64 n, double value) {
+    const __hip_bfloat16 v = __float2bfloat16((float)value);
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) out[i] = v;
+}
+
 // fill_buffer_adv (src/tools/init.hpp:31-46)
 template <class T>
 __global__ void fill_sin4_kernel(T *data, AdvDev p) {
@@ -390,6 +400,17 @@ extern "C" bkma_status bkma_fill_normal(bkma_dtype dtype, void *out, uint64_t n,
         hipLaunchKernelGGL(fill_normal_kernel<T>, dim3(grid_for((i64)n, 256)), dim3(256), 0, as_stream(stream),
                            (T *)out, (i64)n, key, stddev);
         return check_launch("fill_normal_kernel");
+    });
+}
+
+extern "C" bkma_status bkma_fill_constant(bkma_dtype dtype, void *out, uint64_t n, double value, void *stream) {
+    if (!out && n) return fail(BKMA_ERR_INVALID_ARGUMENT, "null output");
+    BKMA_TRY(require_device());
+    return with_dtype(dtype, [&](auto *tag) -> bkma_status {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL(fill_constant_kernel<T>, dim3(grid_for((i64)n, 256)), dim3(256), 0, as_stream(stream),
+                           (T *)out, (i64)n, value);
+        return check_launch("fill_constant_kernel");
     });
 }
 
