@@ -213,6 +213,38 @@ void bkma_adv_plan_destroy(bkma_adv_plan *plan);
 int32_t bkma_adv_plan_variants(bkma_dtype dtype, bkma_tuned_config *out, int32_t max_out);
 
 /* ------------------------------------------------------------------ */
+/* Ragged batches of advection problems                                */
+/* ------------------------------------------------------------------ */
+/* Many independent problems (each its own [n0][n1][n2] array, dt and
+ * velocity grid; the reference runs one bkma_run per problem and step)
+ * advanced together: every step is one persistent launch over all
+ * problems' work items (plus a halo-snapshot launch when a problem's lines
+ * exceed the LDS budget), and bkma_adv_batch_run(use_graph = 1) captures the
+ * nsteps steps into one hipGraph, replayed on later calls with the same
+ * nsteps.  Results are bit-identical to running each problem's steps alone
+ * (FP64: the reference's arithmetic; FP32: the tuned FP32 arithmetic). */
+typedef struct bkma_adv_problem {
+    void *data;              /* device array [n0][n1][n2] of the batch dtype, 16-byte aligned */
+    bkma_adv_params params;  /* update_deltas applied */
+} bkma_adv_problem;
+
+typedef struct bkma_adv_batch bkma_adv_batch;
+
+/* autotune = 1 times the LDS budgets (32 KB .. the device limit) on the
+ * problems' own arrays, which the timing steps modify: create the batch
+ * before initialising the data. */
+bkma_status bkma_adv_batch_create(const bkma_adv_problem *problems, int32_t nproblems, bkma_dtype dtype,
+                                  int32_t autotune, void *stream, bkma_adv_batch **out);
+bkma_status bkma_adv_batch_run(bkma_adv_batch *batch, uint64_t nsteps, int32_t use_graph, void *stream);
+/* Baselines over the same problems: mode 0 = the reference's loop of
+ * per-problem bkma_run<AdvectionSolver, AdaptiveWg> launches with
+ * create_optim_params (pref_wg_size 256); mode 1 = per-problem tuned plans. */
+bkma_status bkma_adv_batch_run_loop(bkma_adv_batch *batch, uint64_t nsteps, int32_t mode, void *stream);
+bkma_status bkma_adv_batch_describe(const bkma_adv_batch *batch, uint64_t *cells, uint64_t *items,
+                                    uint64_t *segmented_items, uint64_t *lds_budget, uint32_t *grid);
+void bkma_adv_batch_destroy(bkma_adv_batch *batch);
+
+/* ------------------------------------------------------------------ */
 /* Harness kernels: fill_buffer_* (src/tools/init.hpp:31-72) and the   */
 /* validators (src/tools/validation.hpp)                                */
 /* ------------------------------------------------------------------ */

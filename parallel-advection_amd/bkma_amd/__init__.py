@@ -104,6 +104,10 @@ STRATEGY_COLUMN_STREAM = 1
 STRATEGY_LINE_LDS = 2
 
 
+class AdvProblem(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("params", AdvParams)]
+
+
 class Span3d(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("e0", ctypes.c_uint64), ("e1", ctypes.c_uint64),
                 ("e2", ctypes.c_uint64)]
@@ -120,7 +124,8 @@ EXPORTS = [
     "bkma_conv_run", "bkma_fill_uniform", "bkma_fill_normal", "bkma_fill_constant", "bkma_fill_adv_sin4",
     "bkma_validate_adv_l1", "bkma_conv_sum_normalize", "bkma_conv_validate", "bkma_error_norms",
     "bkma_adv_plan_create", "bkma_adv_plan_create_with", "bkma_adv_plan_describe", "bkma_adv_plan_run",
-    "bkma_adv_plan_destroy", "bkma_adv_plan_variants",
+    "bkma_adv_plan_destroy", "bkma_adv_plan_variants", "bkma_adv_batch_create", "bkma_adv_batch_run",
+    "bkma_adv_batch_run_loop", "bkma_adv_batch_describe", "bkma_adv_batch_destroy",
 ]
 
 
@@ -160,6 +165,11 @@ def lib():
         "bkma_adv_plan_run": (st, [vp, vp, u64, vp]),
         "bkma_adv_plan_destroy": (None, [vp]),
         "bkma_adv_plan_variants": (i32, [ctypes.c_int, P(TunedConfig), i32]),
+        "bkma_adv_batch_create": (st, [P(AdvProblem), i32, ctypes.c_int, i32, vp, P(vp)]),
+        "bkma_adv_batch_run": (st, [vp, u64, i32, vp]),
+        "bkma_adv_batch_run_loop": (st, [vp, u64, i32, vp]),
+        "bkma_adv_batch_describe": (st, [vp, P(u64), P(u64), P(u64), P(u64), P(ctypes.c_uint32)]),
+        "bkma_adv_batch_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -338,6 +348,45 @@ class AdvPlan:
     def close(self) -> None:
         if self._h:
             lib().bkma_adv_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class AdvBatch:
+    """A ragged batch of advection problems (bkma_adv_batch_*).  `arrays` are
+    contiguous device tensors, `params` their AdvParams."""
+
+    def __init__(self, arrays, params, dtype, autotune: bool = False, stream=None):
+        n = len(arrays)
+        probs = (AdvProblem * n)()
+        for j, (a, p) in enumerate(zip(arrays, params)):
+            probs[j].data = _ptr(a).value
+            probs[j].params = p
+        self.arrays = list(arrays)
+        self._h = ctypes.c_void_p()
+        check(lib().bkma_adv_batch_create(probs, n, dtype, int(autotune), _stream(stream), ctypes.byref(self._h)))
+
+    def run(self, nsteps: int, graph: bool = True, stream=None) -> None:
+        check(lib().bkma_adv_batch_run(self._h, nsteps, int(graph), _stream(stream)))
+
+    def run_loop(self, nsteps: int, mode: int, stream=None) -> None:
+        check(lib().bkma_adv_batch_run_loop(self._h, nsteps, mode, _stream(stream)))
+
+    def describe(self):
+        v = [ctypes.c_uint64() for _ in range(4)]
+        g = ctypes.c_uint32()
+        check(lib().bkma_adv_batch_describe(self._h, *[ctypes.byref(x) for x in v], ctypes.byref(g)))
+        return {"cells": v[0].value, "items": v[1].value, "segmented_items": v[2].value,
+                "lds_budget": v[3].value, "grid": g.value}
+
+    def close(self) -> None:
+        if self._h:
+            lib().bkma_adv_batch_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

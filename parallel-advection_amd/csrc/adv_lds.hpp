@@ -60,22 +60,17 @@ __global__ void halo_snapshot_kernel(const T *__restrict__ data, T *__restrict__
     }
 }
 
+// One LDS-staged unit: rows [r0, r1) of slab i0 (all n2 columns).  Every
+// thread of the work-group calls it; it contains one barrier and no early
+// return, so it can run inside loops of a persistent kernel.
 template <class T, bool SEG>
-__global__ __launch_bounds__(256) void adv_line_lds_kernel(T *__restrict__ data, const T *__restrict__ halo,
-                                                           LdsArgs a) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    T *lds = reinterpret_cast<T *>(lds_raw);
-    const AdvDev &p = a.p;
+__device__ __forceinline__ void line_lds_unit(T *__restrict__ slab, const T *__restrict__ hsrc, const AdvDev &p, i64 i0,
+                                              i64 r0, i64 r1, i64 hb_max, i64 ha_max, T *__restrict__ lds) {
     const i64 n1 = p.n1, n2 = p.n2;
-    const i64 unit = blockIdx.x;
-    const i64 i0 = unit / a.nseg, seg = unit - i0 * a.nseg;
-    const i64 r0 = seg * a.seg_rows;
-    const i64 r1 = min(n1, r0 + a.seg_rows);
     const i64 nr = r1 - r0;
-    T *slab = data + i0 * n1 * n2;
-
+    const int nthr = blockDim.x;
     int hb = 0, ha = 0;
-    if constexpr (SEG) slab_halo(p, i0, a.hb_max, a.ha_max, hb, ha);
+    if constexpr (SEG) slab_halo(p, i0, hb_max, ha_max, hb, ha);
     const i64 lds_rows = hb + nr + ha;
 
     // Stage: the segment (contiguous in HBM), then the halo rows.
@@ -88,30 +83,27 @@ __global__ __launch_bounds__(256) void adv_line_lds_kernel(T *__restrict__ data,
         if (vec_ok) {
             using VT = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
             const i64 nv = cnt / V;
-            for (i64 e = threadIdx.x; e < nv; e += blockDim.x)
+            for (i64 e = threadIdx.x; e < nv; e += nthr)
                 reinterpret_cast<VT *>(dst)[e] = reinterpret_cast<const VT *>(src)[e];
-            for (i64 e = nv * V + threadIdx.x; e < cnt; e += blockDim.x) dst[e] = src[e];
+            for (i64 e = nv * V + threadIdx.x; e < cnt; e += nthr) dst[e] = src[e];
         } else {
-            for (i64 e = threadIdx.x; e < cnt; e += blockDim.x) dst[e] = src[e];
+            for (i64 e = threadIdx.x; e < cnt; e += nthr) dst[e] = src[e];
         }
         if constexpr (SEG) {
-            const T *hsrc = halo + unit * (a.hb_max + a.ha_max) * n2;
-            for (i64 e = threadIdx.x; e < (i64)hb * n2; e += blockDim.x) lds[e] = hsrc[e];
-            for (i64 e = threadIdx.x; e < (i64)ha * n2; e += blockDim.x)
-                lds[(hb + nr) * n2 + e] = hsrc[(i64)hb * n2 + e];
+            for (i64 e = threadIdx.x; e < (i64)hb * n2; e += nthr) lds[e] = hsrc[e];
+            for (i64 e = threadIdx.x; e < (i64)ha * n2; e += nthr) lds[(hb + nr) * n2 + e] = hsrc[(i64)hb * n2 + e];
         }
     }
     __syncthreads();
 
-    // Compute: thread (row offset, column); columns beyond 256 loop.
-    const int cw = (int)min<i64>(n2, 256);
-    const int rows_per_pass = 256 / cw;
+    // Compute: thread (row offset, column); columns beyond nthr loop.
+    const int cw = (int)min<i64>(n2, nthr);
+    const int rows_per_pass = nthr / cw;
     const int t_row = threadIdx.x / cw, t_col = threadIdx.x - t_row * cw;
-    if (t_row >= rows_per_pass) return;  // no barrier follows
     const int half = (int)(n1 / 2);
     double coef[6];
     unsigned long long cached = ~0ull;
-    for (i64 rr = t_row; rr < nr; rr += rows_per_pass) {
+    for (i64 rr = t_row; t_row < rows_per_pass && rr < nr; rr += rows_per_pass) {
         const i64 i1 = r0 + rr;
         double d_prev1;
         const int leftNode = adv_foot(p, (int)i0, (int)i1, d_prev1);
@@ -121,7 +113,7 @@ __global__ __launch_bounds__(256) void adv_line_lds_kernel(T *__restrict__ data,
             cached = bits;
         }
         const int ipos1 = leftNode - 2;
-        i64 lrow;      // LDS row of the first stencil point
+        i64 lrow;  // LDS row of the first stencil point
         bool ok = true;
         if constexpr (SEG) {
             int delta = ipos1 - (int)i1;
@@ -148,6 +140,18 @@ __global__ __launch_bounds__(256) void adv_line_lds_kernel(T *__restrict__ data,
             slab[i1 * n2 + c] = (T)acc;
         }
     }
+}
+
+template <class T, bool SEG>
+__global__ __launch_bounds__(256) void adv_line_lds_kernel(T *__restrict__ data, const T *__restrict__ halo,
+                                                           LdsArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const i64 unit = blockIdx.x;
+    const i64 i0 = unit / a.nseg, seg = unit - i0 * a.nseg;
+    const i64 r0 = seg * a.seg_rows;
+    const i64 r1 = min(a.p.n1, r0 + a.seg_rows);
+    line_lds_unit<T, SEG>(data + i0 * a.p.n1 * a.p.n2, SEG ? halo + unit * (a.hb_max + a.ha_max) * a.p.n2 : nullptr,
+                          a.p, i0, r0, r1, a.hb_max, a.ha_max, reinterpret_cast<T *>(lds_raw));
 }
 
 }  // namespace bkma
