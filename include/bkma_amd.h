@@ -245,6 +245,40 @@ bkma_status bkma_adv_batch_describe(const bkma_adv_batch *batch, uint64_t *cells
 void bkma_adv_batch_destroy(bkma_adv_batch *batch);
 
 /* ------------------------------------------------------------------ */
+/* Conv1d, textbook semantics (torch.nn.functional.conv1d, stride 1,   */
+/* no padding), in place                                                */
+/* ------------------------------------------------------------------ */
+/* x[B][C][L] (contiguous, the reference's [n0][n1 = C*L][n2 = 1] BKMA
+ * layout), weight[C_out = C][C_in = C][K], bias[C].  One run writes
+ * y[b][oc][j], j < L-K+1, over x[b][oc][j] and keeps x[b][oc][L-K+1 ..).
+ * Replaces the ConvSolver call of the reference's conv1d driver
+ * (src/conv1d.cpp:83-87) for callers that want the textbook contraction
+ * (the reference functor's own semantics stay available as bkma_conv_run).
+ *  BKMA_CONV_F64   : io F64, weights F64; FP64, the oracle's summation order
+ *  BKMA_CONV_F32   : io F32, weights F32; FP32 fused multiply-add
+ *  BKMA_CONV_BF16  : io F32 or BF16, weights F32; bf16 MFMA, FP32 accumulate
+ *  BKMA_CONV_BF16X3: io F32 or BF16, weights F32; split bf16 (hi*hi + hi*lo
+ *                    + lo*hi) MFMA, FP32 accumulate */
+typedef enum bkma_conv_mode {
+    BKMA_CONV_F64 = 0,
+    BKMA_CONV_F32 = 1,
+    BKMA_CONV_BF16 = 2,
+    BKMA_CONV_BF16X3 = 3
+} bkma_conv_mode;
+
+typedef struct bkma_conv1d_plan bkma_conv1d_plan;
+
+bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype io_dtype, uint64_t B, uint64_t C, uint64_t L,
+                                    uint64_t K, bkma_dtype w_dtype, const void *weight, const void *bias,
+                                    void *stream, bkma_conv1d_plan **out);
+bkma_status bkma_conv1d_plan_run(bkma_conv1d_plan *plan, void *x, void *stream);
+bkma_status bkma_conv1d_plan_describe(const bkma_conv1d_plan *plan, char *name, uint64_t name_len);
+void bkma_conv1d_plan_destroy(bkma_conv1d_plan *plan);
+/* One v_mfma_f32_32x32x16_bf16 on lane fragments a[64][8], b[64][8] (bf16)
+ * into d[64][16] (f32): the check of the operand layout the conv kernels use. */
+bkma_status bkma_mfma_layout_probe(const void *a, const void *b, void *d, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Harness kernels: fill_buffer_* (src/tools/init.hpp:31-72) and the   */
 /* validators (src/tools/validation.hpp)                                */
 /* ------------------------------------------------------------------ */

@@ -27,6 +27,7 @@ BKMA_ERR_OUT_OF_MEMORY = 5
 BASIC_RANGE, ND_RANGE, ADAPTIVE_WG = 0, 1, 2
 MEM_LOCAL, MEM_GLOBAL = 0, 1
 F64, F32, BF16 = 0, 1, 2
+CONV_F64, CONV_F32, CONV_BF16, CONV_BF16X3 = 0, 1, 2, 3
 
 
 class BkmaError(RuntimeError):
@@ -126,6 +127,8 @@ EXPORTS = [
     "bkma_adv_plan_create", "bkma_adv_plan_create_with", "bkma_adv_plan_describe", "bkma_adv_plan_run",
     "bkma_adv_plan_destroy", "bkma_adv_plan_variants", "bkma_adv_batch_create", "bkma_adv_batch_run",
     "bkma_adv_batch_run_loop", "bkma_adv_batch_describe", "bkma_adv_batch_destroy",
+    "bkma_conv1d_plan_create", "bkma_conv1d_plan_run", "bkma_conv1d_plan_describe", "bkma_conv1d_plan_destroy",
+    "bkma_mfma_layout_probe",
 ]
 
 
@@ -170,6 +173,12 @@ def lib():
         "bkma_adv_batch_run_loop": (st, [vp, u64, i32, vp]),
         "bkma_adv_batch_describe": (st, [vp, P(u64), P(u64), P(u64), P(u64), P(ctypes.c_uint32)]),
         "bkma_adv_batch_destroy": (None, [vp]),
+        "bkma_conv1d_plan_create": (st, [ctypes.c_int, ctypes.c_int, u64, u64, u64, u64, ctypes.c_int, vp, vp, vp,
+                                         P(vp)]),
+        "bkma_conv1d_plan_run": (st, [vp, vp, vp]),
+        "bkma_conv1d_plan_describe": (st, [vp, ctypes.c_char_p, u64]),
+        "bkma_conv1d_plan_destroy": (None, [vp]),
+        "bkma_mfma_layout_probe": (st, [vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -394,3 +403,36 @@ class AdvBatch:
             self.close()
         except Exception:
             pass
+
+
+class Conv1dPlan:
+    """Textbook in-place Conv1d (bkma_conv1d_plan_*): x[B][C][L] device tensor,
+    weight[C][C][K] and bias[C] device tensors (F64 for CONV_F64, else F32)."""
+
+    def __init__(self, mode, io_dtype, B, C, L, K, weight, bias, stream=None):
+        self._h = ctypes.c_void_p()
+        check(lib().bkma_conv1d_plan_create(mode, io_dtype, B, C, L, K, dtype_code(weight), _ptr(weight),
+                                            _ptr(bias), _stream(stream), ctypes.byref(self._h)))
+
+    def run(self, x, stream=None) -> None:
+        check(lib().bkma_conv1d_plan_run(self._h, _ptr(x), _stream(stream)))
+
+    def name(self) -> str:
+        buf = ctypes.create_string_buffer(128)
+        check(lib().bkma_conv1d_plan_describe(self._h, buf, 128))
+        return buf.value.decode()
+
+    def close(self) -> None:
+        if self._h:
+            lib().bkma_conv1d_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mfma_layout_probe(a, b, d, stream=None) -> None:
+    check(lib().bkma_mfma_layout_probe(_ptr(a), _ptr(b), _ptr(d), _stream(stream)))
