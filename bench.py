@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "adv_column_stream_f32_2048x1024x512.traffic.json"))
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (3 launches), for scaling rocprofv3 byte counters")
+    ap.add_argument("--extra", default="",
+                    help="comma list of extra measurements added to the JSON line: conv, batch")
     return ap.parse_args()
 
 
@@ -86,6 +90,15 @@ def main():
 
     import torch
     import bkma_amd as B
+
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4, "launches": 3}))
+        return
 
     n0 = args.n0
     dev = torch.device("cuda:0")
@@ -185,7 +198,140 @@ def main():
                                "after_steps": args.warmup + args.steps},
         "cpu_baseline": cpu,
     }
+    del f64, f32
+    torch.cuda.empty_cache()
+    extras = [e for e in args.extra.split(",") if e]
+    if "conv" in extras:
+        out["conv1d"] = measure_conv(torch, B)
+    if "batch" in extras:
+        out["ragged_batch"] = measure_batch(torch, B)
     print(json.dumps(out), flush=True)
+
+
+def measure_conv(torch, B):
+    """Conv1d x[1024][128][4096], w[128][128][5] (seed 2): time per call and
+    TFLOP/s of each mode; errors of each mode against the FP64 GPU plan (which
+    is bit-identical to the oracle) on the same inputs."""
+    Bn, C, L, K = 1024, 128, 4096, 5
+    flop = 2.0 * Bn * C * C * K * (L - K + 1)
+    w = torch.empty((C, C, K), dtype=torch.float64, device="cuda")
+    B.fill_normal(w, 21, 1.0 / (C * K) ** 0.5)
+    b = torch.zeros(C, dtype=torch.float64, device="cuda")
+    x64 = torch.empty((Bn, C, L), dtype=torch.float64, device="cuda")
+    B.fill_normal(x64, 2)
+    x32 = x64.float()
+    ref = x64.clone()
+    p64 = B.Conv1dPlan(B.CONV_F64, B.F64, Bn, C, L, K, w, b)
+    t0 = time.perf_counter()
+    p64.run(ref)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    res = {"workload": f"x[{Bn}][{C}][{L}] w[{C}][{C}][{K}] in place", "flop_per_call": flop,
+           "f64": {"ms": round(t64 * 1e3, 3), "TFLOPs": round(flop / t64 / 1e12, 3)}}
+    del x64
+    wf, bf = w.float(), b.float()
+    for name, mode, io in (("f32", B.CONV_F32, B.F32), ("bf16_io_f32", B.CONV_BF16, B.F32),
+                           ("bf16x3_io_f32", B.CONV_BF16X3, B.F32), ("bf16_io_bf16", B.CONV_BF16, B.BF16),
+                           ("bf16x3_io_bf16", B.CONV_BF16X3, B.BF16)):
+        try:
+            plan = B.Conv1dPlan(mode, io, Bn, C, L, K, wf, bf)
+        except B.BkmaError as e:
+            res[name] = {"error": str(e)}
+            continue
+        xin = x32 if io == B.F32 else x32.to(torch.bfloat16)
+        x = xin.clone()
+        plan.run(x)  # warm-up and accuracy run
+        torch.cuda.synchronize()
+        # error vs FP64 on the values this mode read (bf16 io rounds x first)
+        ref_in = xin.double()
+        r2 = ref_in.clone()
+        p64.run(r2)
+        Lout = L - K + 1
+        got = x[:, :, :Lout].double()
+        exp = r2[:, :, :Lout]
+        d = (got - exp)
+        rel_max = (d.abs().max() / exp.abs().max()).item()
+        rel_l2 = (d.norm() / exp.norm()).item()
+        del r2, got, exp, d
+        reps = 5
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            plan.run(x)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        peak = 2500.0 if mode in (B.CONV_BF16, B.CONV_BF16X3) else 157.3
+        res[name] = {"kernel": plan.name(), "ms": round(ms, 4), "TFLOPs": round(flop / ms / 1e9, 2),
+                     "frac_of_dense_bf16_peak" if peak == 2500.0 else "frac_of_fp32_peak":
+                         round(flop / ms / 1e9 / peak, 4),
+                     "rel_max": rel_max, "rel_L2": rel_l2}
+        plan.close()
+        del x
+        torch.cuda.empty_cache()
+    return res
+
+
+def measure_batch(torch, B):
+    """64 ragged problems (seed 3): n1 log-uniform in [64, 16384] (multiple of
+    64), n0 uniform in [16, 1024], n2 in {1, 8, 64}, dt in [5e-4, 2e-3], FP32,
+    200 steps; persistent graph-captured batch vs loops of per-problem launches."""
+    import math
+    sys.path.insert(0, os.path.join(ROOT, "parallel-advection_amd"))
+    specs = ragged_specs(3, 64)
+    arrays = [torch.empty((n0, n1, n2), dtype=torch.float32, device="cuda") for n0, n1, n2, _ in specs]
+    params = [B.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1) for n0, n1, n2, dt in specs]
+    batch = B.AdvBatch(arrays, params, B.F32, autotune=True)
+    info = batch.describe()
+    for j, a in enumerate(arrays):
+        B.fill_uniform(a, 3000 + j)
+    steps = 200
+    cells = info["cells"]
+    out = {"problems": len(specs), "cells": cells, "steps": steps, **info}
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    batch.run(steps, graph=True)  # capture + first replay
+    tg = timed(lambda: batch.run(steps, graph=True))
+    out["graph_persistent"] = {"s": round(tg, 4), "Gcell_s": round(cells * steps / tg / 1e9, 2),
+                               "GBps_alg": round(cells * steps * 8 / tg / 1e9, 1)}
+    te = timed(lambda: batch.run(steps, graph=False))
+    out["persistent_no_graph"] = {"s": round(te, 4), "Gcell_s": round(cells * steps / te / 1e9, 2)}
+    batch.run_loop(1, 1)
+    tl = timed(lambda: batch.run_loop(steps, 1))
+    out["loop_tuned_per_problem"] = {"s": round(tl, 4), "Gcell_s": round(cells * steps / tl / 1e9, 2)}
+    steps_ref = 20  # the reference loop is slow: 20 steps, rate scaled
+    tr = timed(lambda: batch.run_loop(steps_ref, 0))
+    out["loop_reference_adaptivewg"] = {"s_per_step": round(tr / steps_ref, 5),
+                                        "Gcell_s": round(cells * steps_ref / tr / 1e9, 2), "steps": steps_ref}
+    out["speedup_vs_reference_loop"] = round((tr / steps_ref) / (tg / steps), 2)
+    out["speedup_vs_tuned_loop"] = round(tl / tg, 3)
+    batch.close()
+    return out
+
+
+def ragged_specs(seed, count, n1_lo=64, n1_hi=16384, n0_lo=16, n0_hi=1024):
+    """Shapes of the ragged batch from the counter-based generator: problem j
+    uses draws 4j..4j+3 of `seed`."""
+    import math
+    import bkma_amd as B  # noqa: F401  (generator constants live in the oracle and the library)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    out = []
+    for j in range(count):
+        u = [O.lib().orc_uniform(seed, 4 * j + k) for k in range(4)]
+        n1 = int(round(math.exp(math.log(n1_lo) + u[0] * (math.log(n1_hi) - math.log(n1_lo))) / 64)) * 64
+        n1 = max(64, min(n1_hi, n1))
+        n0 = n0_lo + int(u[1] * (n0_hi - n0_lo + 1))
+        n2 = (1, 8, 64)[min(2, int(u[2] * 3))]
+        dt = 5e-4 + u[3] * 1.5e-3
+        out.append((n0, n1, n2, dt))
+    return out
 
 
 if __name__ == "__main__":
