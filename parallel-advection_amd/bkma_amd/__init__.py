@@ -140,6 +140,10 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                           f"(no CPU fallback exists for the BKMA kernels)")
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7).
+    # Load it first so that libbkma_amd.so binds to that copy: two HIP
+    # runtimes in one process do not see the same devices.
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIB_PATH)
     P, u64, i32, vp, dbl = ctypes.POINTER, ctypes.c_uint64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_double
     st = ctypes.c_int

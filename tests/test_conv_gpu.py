@@ -72,9 +72,11 @@ def _rel(got, ref, Lout):
     return np.abs(d).max() / np.abs(ref[:, :, :Lout]).max(), np.linalg.norm(d) / np.linalg.norm(ref[:, :, :Lout])
 
 
+# bf16 storage rounds every output to 8 bits: rel_max is then bounded by the
+# output rounding (~2^-8 of the largest output), whatever the products' precision.
 @pytest.mark.parametrize("mode,io,tol", [(B.CONV_F32, B.F32, 1e-6), (B.CONV_BF16, B.F32, 1e-2),
                                          (B.CONV_BF16X3, B.F32, 1e-4), (B.CONV_BF16, B.BF16, 1e-2),
-                                         (B.CONV_BF16X3, B.BF16, 1e-4)],
+                                         (B.CONV_BF16X3, B.BF16, 8e-3)],
                          ids=["f32", "bf16_io32", "bf16x3_io32", "bf16_io16", "bf16x3_io16"])
 @pytest.mark.parametrize("K", [5, 3])
 def test_conv_reduced_precision(cuda, mode, io, tol, K):
