@@ -191,6 +191,8 @@ typedef struct bkma_tuned_config {
     int32_t nontemporal;  /* 1: non-temporal global loads/stores */
     int32_t persistent;   /* 1: grid capped at the resident work-groups, waves loop over units */
     int32_t seg_rows;     /* LINE_LDS: rows per segment (0 = whole line in LDS) */
+    int32_t wide_window;  /* COLUMN_STREAM, F32 storage: 1 = FP64 register window (rows widened once) */
+    int32_t _reserved[3];
 } bkma_tuned_config;
 
 typedef struct bkma_adv_plan bkma_adv_plan;

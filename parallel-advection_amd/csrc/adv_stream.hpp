@@ -66,28 +66,28 @@ template <int SMAX, int P> struct StreamShape {
     static constexpr int U = clcm(WS, P);   // unrolled rows per loop trip
 };
 
-// 6-tap interpolation of VEC cells from the FP64 window slots S0..S0+5
-// (mod WS).  FP64 storage: the reference's separately rounded multiply and
+// 6-tap interpolation of VEC cells from the window slots S0..S0+5 (mod WS),
+// in FP64.  FP64 storage: the reference's separately rounded multiply and
 // add; FP32 storage: fused FP64 multiply-add.
-template <class T, int VEC, int WS, int S0>
-__device__ __forceinline__ void interp_window(const double (&win)[WS][VEC], const double (&c)[6], T (&out)[VEC]) {
+template <class T, class WT, int VEC, int WS, int S0>
+__device__ __forceinline__ void interp_window(const WT (&win)[WS][VEC], const double (&c)[6], T (&out)[VEC]) {
 #pragma unroll
     for (int v = 0; v < VEC; ++v) {
         double acc = 0.;
         if constexpr (sizeof(T) == 8) {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) acc = acc + c[k] * win[(S0 + k) % WS][v];
+            for (int k = 0; k < 6; ++k) acc = acc + c[k] * (double)win[(S0 + k) % WS][v];
         } else {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) acc = fma(c[k], win[(S0 + k) % WS][v], acc);
+            for (int k = 0; k < 6; ++k) acc = fma(c[k], (double)win[(S0 + k) % WS][v], acc);
         }
         out[v] = (T)acc;
     }
 }
 
 // Select the window slots for the (wave-uniform) shift d at loop position J.
-template <class T, int VEC, int WS, int J, int BACK, int D, int DHI>
-__device__ __forceinline__ void interp_shift(int d, const double (&win)[WS][VEC], const double (&c)[6],
+template <class T, class WT, int VEC, int WS, int J, int BACK, int D, int DHI>
+__device__ __forceinline__ void interp_shift(int d, const WT (&win)[WS][VEC], const double (&c)[6],
                                              T (&out)[VEC]) {
     if constexpr (D > DHI) {
 #pragma unroll
@@ -95,9 +95,9 @@ __device__ __forceinline__ void interp_shift(int d, const double (&win)[WS][VEC]
     } else {
         if (d == D) {
             constexpr int S0 = (((J + BACK - D - 2) % WS) + WS) % WS;
-            interp_window<T, VEC, WS, S0>(win, c, out);
+            interp_window<T, WT, VEC, WS, S0>(win, c, out);
         } else {
-            interp_shift<T, VEC, WS, J, BACK, D + 1, DHI>(d, win, c, out);
+            interp_shift<T, WT, VEC, WS, J, BACK, D + 1, DHI>(d, win, c, out);
         }
     }
 }
@@ -110,9 +110,11 @@ struct StreamArgs {
 };
 
 // One wavefront streams unit (i0, column tile) through the whole line.
-// Row r of the line lives, in turn, in: the FP32/FP64 staging ring (loaded,
-// P rows ahead), the FP64 window (rows q-BACK .. q+AHEAD around output q).
-template <class T, int VEC, int SMAX, int P, bool NT>
+// Row r of the line lives, in turn, in: the staging ring (loaded, P rows
+// ahead), the register window (rows q-BACK .. q+AHEAD around output q).
+// WT: window element type -- double (FP32 rows widened once, on entry) or
+// the storage type (widened at each use; fewer registers).
+template <class T, class WT, int VEC, int SMAX, int P, bool NT>
 __device__ __forceinline__ void stream_unit(T *__restrict__ data, const StreamArgs &a, i64 unit, int lane,
                                             T (*head)[64 * VEC]) {
     using S = StreamShape<SMAX, P>;
@@ -126,7 +128,7 @@ __device__ __forceinline__ void stream_unit(T *__restrict__ data, const StreamAr
     const bool lane_ok = col < n2;  // VEC divides n2: the whole vector is in range
     T *slab = data + (i0 * n1) * n2 + (lane_ok ? col : 0);
 
-    double win[WS][VEC];
+    WT win[WS][VEC];
     T stg[P][VEC];
     auto load_row = [&](i64 row, T (&dst)[VEC]) __attribute__((always_inline)) {
         if (lane_ok) {
@@ -146,7 +148,7 @@ __device__ __forceinline__ void stream_unit(T *__restrict__ data, const StreamAr
         T tmp[VEC];
         load_row(r < 0 ? n1 + r : r, tmp);
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) win[s][v] = (double)tmp[v];
+        for (int v = 0; v < VEC; ++v) win[s][v] = (WT)tmp[v];
         if constexpr (r >= 0) {
 #pragma unroll
             for (int v = 0; v < VEC; ++v) head[r][lane * VEC + v] = tmp[v];
@@ -173,7 +175,7 @@ __device__ __forceinline__ void stream_unit(T *__restrict__ data, const StreamAr
             const i64 ra = q + AHEAD;
             if (ra < n1) {
 #pragma unroll
-                for (int v = 0; v < VEC; ++v) win[sw][v] = (double)stg[ss][v];
+                for (int v = 0; v < VEC; ++v) win[sw][v] = (WT)stg[ss][v];
                 // 2. refill that staging slot with row q+AHEAD+P
                 const i64 rn = ra + P;
                 if (rn < n1) load_row(rn, stg[ss]);
@@ -183,7 +185,7 @@ __device__ __forceinline__ void stream_unit(T *__restrict__ data, const StreamAr
                 for (int hh = 0; hh < AHEAD; ++hh)
                     if (h == hh) {
 #pragma unroll
-                        for (int v = 0; v < VEC; ++v) win[sw][v] = (double)head[hh][lane * VEC + v];
+                        for (int v = 0; v < VEC; ++v) win[sw][v] = (WT)head[hh][lane * VEC + v];
                     }
             }
         }
@@ -203,14 +205,14 @@ __device__ __forceinline__ void stream_unit(T *__restrict__ data, const StreamAr
         if (d < -half) d += (int)n1;
         // 4. stencil rows q-d-2 .. q-d+3 from the window, 5. store row q
         T out[VEC];
-        interp_shift<T, VEC, WS, J, BACK, S::DLO, S::DHI>(d, win, coef, out);
+        interp_shift<T, WT, VEC, WS, J, BACK, S::DLO, S::DHI>(d, win, coef, out);
         if (lane_ok) M::store(slab + q * n2, out);
     };
 
     for (; base < n1; base += U) static_for<0, U>(row_step);
 }
 
-template <class T, int VEC, int SMAX, int P, int WPB, bool NT>
+template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT>
 __global__ __launch_bounds__(64 * WPB) void adv_column_stream_kernel(T *__restrict__ data, StreamArgs a) {
     using S = StreamShape<SMAX, P>;
     __shared__ T head_lds[WPB][S::AHEAD][64 * VEC];
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(64 * WPB) void adv_column_stream_kernel(T *__restri
     // Persistent waves: unit = wave, wave + stride, ...  (all lanes of a wave
     // take the same trip count; there is no work-group barrier).
     for (i64 unit = (i64)blockIdx.x * WPB + wv; unit < a.total_units; unit += a.wave_stride)
-        stream_unit<T, VEC, SMAX, P, NT>(data, a, unit, lane, head_lds[wv]);
+        stream_unit<T, WT, VEC, SMAX, P, NT>(data, a, unit, lane, head_lds[wv]);
 }
 
 }  // namespace bkma

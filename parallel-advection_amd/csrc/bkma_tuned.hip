@@ -38,16 +38,16 @@ namespace {
 using LaunchFn = void (*)(void *data, const StreamArgs &a, unsigned grid, hipStream_t s);
 using OccFn = int (*)();
 
-template <class T, int VEC, int SMAX, int P, int WPB, bool NT>
+template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT>
 void launch_stream(void *data, const StreamArgs &a, unsigned grid, hipStream_t s) {
-    hipLaunchKernelGGL((adv_column_stream_kernel<T, VEC, SMAX, P, WPB, NT>), dim3(grid), dim3(64 * WPB), 0, s,
+    hipLaunchKernelGGL((adv_column_stream_kernel<T, WT, VEC, SMAX, P, WPB, NT>), dim3(grid), dim3(64 * WPB), 0, s,
                        reinterpret_cast<T *>(data), a);
 }
 
 // Resident work-groups per CU for a variant (0 if the query fails).
-template <class T, int VEC, int SMAX, int P, int WPB, bool NT> int occupancy_stream() {
+template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT> int occupancy_stream() {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, adv_column_stream_kernel<T, VEC, SMAX, P, WPB, NT>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, adv_column_stream_kernel<T, WT, VEC, SMAX, P, WPB, NT>,
                                                      64 * WPB, 0) != hipSuccess)
         return 0;
     return blocks;
@@ -55,28 +55,30 @@ template <class T, int VEC, int SMAX, int P, int WPB, bool NT> int occupancy_str
 
 struct StreamVariant {
     bkma_dtype dtype;
-    int vec, smax, prefetch, wpb, nt;
+    int vec, smax, prefetch, wpb, nt, wide;  // wide: FP64 window for FP32 storage
     LaunchFn fn;
     OccFn occ;
 };
 
-#define SV(T, DT, V, S, P, W, NT) \
-    StreamVariant{DT, V, S, P, W, NT, &launch_stream<T, V, S, P, W, NT>, &occupancy_stream<T, V, S, P, W, NT>}
+#define SV(T, WT, DT, V, S, P, W, NT)                                                                      \
+    StreamVariant{DT, V, S, P, W, NT, sizeof(WT) > sizeof(T), &launch_stream<T, WT, V, S, P, W, NT>,       \
+                  &occupancy_stream<T, WT, V, S, P, W, NT>}
 // P is WS/2 or WS (WS = 2*SMAX + 6) so the unrolled loop stays one window long.
 const StreamVariant kStreamVariants[] = {
 #ifdef BKMA_TUNED_FEW
-    SV(float, BKMA_F32, 4, 2, 10, 1, false),
+    SV(float, float, BKMA_F32, 4, 2, 10, 1, true),
 #else
-    SV(float, BKMA_F32, 4, 2, 5, 1, false),   SV(float, BKMA_F32, 4, 2, 10, 1, false),
-    SV(float, BKMA_F32, 4, 2, 10, 1, true),   SV(float, BKMA_F32, 2, 2, 10, 1, false),
-    SV(float, BKMA_F32, 2, 2, 10, 1, true),   SV(float, BKMA_F32, 1, 2, 10, 1, false),
-    SV(float, BKMA_F32, 4, 1, 8, 1, false),   SV(float, BKMA_F32, 2, 1, 8, 1, false),
-    SV(float, BKMA_F32, 4, 4, 7, 1, false),   SV(float, BKMA_F32, 2, 4, 14, 1, false),
-    SV(float, BKMA_F32, 1, 4, 14, 1, false),
-    SV(double, BKMA_F64, 2, 2, 5, 1, false),  SV(double, BKMA_F64, 2, 2, 10, 1, false),
-    SV(double, BKMA_F64, 2, 2, 10, 1, true),  SV(double, BKMA_F64, 1, 2, 10, 1, false),
-    SV(double, BKMA_F64, 2, 1, 8, 1, false),  SV(double, BKMA_F64, 2, 4, 7, 1, false),
-    SV(double, BKMA_F64, 1, 4, 14, 1, false),
+    SV(float, double, BKMA_F32, 4, 2, 5, 1, true),  SV(float, double, BKMA_F32, 4, 2, 10, 1, true),
+    SV(float, float, BKMA_F32, 4, 2, 5, 1, true),   SV(float, float, BKMA_F32, 4, 2, 10, 1, true),
+    SV(float, float, BKMA_F32, 4, 2, 10, 1, false), SV(float, double, BKMA_F32, 2, 2, 10, 1, true),
+    SV(float, float, BKMA_F32, 2, 2, 10, 1, true),  SV(float, double, BKMA_F32, 1, 2, 10, 1, false),
+    SV(float, float, BKMA_F32, 4, 1, 8, 1, true),   SV(float, double, BKMA_F32, 4, 1, 8, 1, true),
+    SV(float, float, BKMA_F32, 2, 1, 8, 1, true),   SV(float, float, BKMA_F32, 4, 4, 7, 1, true),
+    SV(float, double, BKMA_F32, 2, 4, 14, 1, true), SV(float, double, BKMA_F32, 1, 4, 14, 1, false),
+    SV(double, double, BKMA_F64, 2, 2, 5, 1, true), SV(double, double, BKMA_F64, 2, 2, 10, 1, true),
+    SV(double, double, BKMA_F64, 2, 2, 5, 1, false), SV(double, double, BKMA_F64, 1, 2, 10, 1, false),
+    SV(double, double, BKMA_F64, 2, 1, 8, 1, true), SV(double, double, BKMA_F64, 2, 4, 7, 1, true),
+    SV(double, double, BKMA_F64, 1, 4, 14, 1, false),
 #endif
 };
 #undef SV
@@ -232,7 +234,8 @@ bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s) {
 }
 
 std::string stream_name(const StreamVariant &v, bool persistent) {
-    return std::string("column_stream<") + (v.dtype == BKMA_F64 ? "f64" : "f32") + ",vec" + std::to_string(v.vec) +
+    return std::string("column_stream<") + (v.dtype == BKMA_F64 ? "f64" : (v.wide ? "f32,win64" : "f32,win32")) +
+           ",vec" + std::to_string(v.vec) +
            ",smax" + std::to_string(v.smax) + ",p" + std::to_string(v.prefetch) + ",wpb" + std::to_string(v.wpb) +
            (v.nt ? ",nt" : "") + (persistent ? ",persistent" : "") + ">";
 }
@@ -263,7 +266,7 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
     if (cfg.strategy == BKMA_STRATEGY_COLUMN_STREAM) {
         for (const auto &v : kStreamVariants) {
             if (v.dtype == dt && v.vec == cfg.vec && v.smax == cfg.smax && v.prefetch == cfg.prefetch &&
-                v.wpb == cfg.waves_per_wg && v.nt == (cfg.nontemporal != 0)) {
+                v.wpb == cfg.waves_per_wg && v.nt == (cfg.nontemporal != 0) && v.wide == (cfg.wide_window != 0)) {
                 if (stream_feasible(p, dt, v, &why) != BKMA_OK)
                     return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
                 pl->variant = &v;
@@ -310,6 +313,7 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
             c.prefetch = v.prefetch;
             c.waves_per_wg = v.wpb;
             c.nontemporal = v.nt;
+            c.wide_window = v.wide;
             c.persistent = pers;
             out.push_back(c);
         }
@@ -393,16 +397,22 @@ extern "C" bkma_status bkma_adv_plan_create(const bkma_adv_params *params, bkma_
     if (autotune && cands.size() > 1) {
         // Time the candidates on a scratch array of whole slabs, large enough
         // to fill the GPU several times over (contents do not affect timing).
+        // The whole problem (up to 2 Gi cells) is used: occupancy and tail
+        // effects depend on the number of work units.
         bkma_adv_params sub = *params;
         const u64 elems_per_slab = params->n1 * params->n2;
-        const u64 target = 256ull << 20;  // ~256 Mi cells
+        const u64 target = 2048ull << 20;
         sub.n0 = std::max<u64>(1, std::min<u64>(params->n0, target / std::max<u64>(elems_per_slab, 1)));
         const size_t bytes = (size_t)(sub.n0 * elems_per_slab) * (dtype == BKMA_F64 ? 8 : 4);
         void *buf = nullptr;
-        BKMA_HIP_TRY(hipMalloc(&buf, bytes));
-        (void)hipMemsetAsync(buf, 0, bytes, as_stream(stream));
+        if (hipMalloc(&buf, bytes) != hipSuccess) {  // no room for the scratch: keep the heuristic choice
+            (void)hipGetLastError();
+            buf = nullptr;
+        }
+        if (buf) (void)hipMemsetAsync(buf, 0, bytes, as_stream(stream));
         float best_ms = 1e30f;
         for (const auto &c : cands) {
+            if (!buf) break;
             TunedPlan pl;
             if (build_plan(sub, dtype, c, &pl) != BKMA_OK) continue;
             float ms = 0;
@@ -478,6 +488,7 @@ extern "C" int32_t bkma_adv_plan_variants(bkma_dtype dtype, bkma_tuned_config *o
             c.prefetch = v.prefetch;
             c.waves_per_wg = v.wpb;
             c.nontemporal = v.nt;
+            c.wide_window = v.wide;
             out[n] = c;
         }
         ++n;

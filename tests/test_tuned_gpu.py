@@ -60,7 +60,8 @@ def test_tuned_f32_bit_exact_vs_restatement(cuda, shape):
 
 
 def _variant_id(c):
-    return f"vec{c.vec}-smax{c.smax}-p{c.prefetch}-wpb{c.waves_per_wg}{'-nt' if c.nontemporal else ''}"
+    return (f"vec{c.vec}-smax{c.smax}-p{c.prefetch}-wpb{c.waves_per_wg}{'-nt' if c.nontemporal else ''}"
+            f"{'-w64' if c.wide_window else ''}")
 
 
 F32_VARIANTS = B.plan_variants(B.F32)
