@@ -174,9 +174,13 @@ bkma_status bkma_conv_run(bkma_impl impl, bkma_dtype dtype, void *data, uint64_t
  * (hand-set in the reference through BkmaOptimParams / AdaptiveWg), then
  * runs any number of in-place steps.
  *  - F64 storage: results bit-identical to bkma_adv_run (the reference's
- *    FP64 arithmetic, reordered only across cells).
+ *    FP64 arithmetic, reordered only across cells), whatever the number of
+ *    steps fused per pass.
  *  - F32 storage: FP64 foot location / index / lag_basis exactly as the
- *    reference, FP64 fused accumulation, one rounding to FP32 per step. */
+ *    reference, FP64 fused accumulation; the steps fused into one pass keep
+ *    their intermediate values in FP64 and round to FP32 once per pass.  A
+ *    run of nsteps makes nsteps / steps_per_pass full passes, then one pass
+ *    of the remaining steps. */
 typedef enum bkma_strategy {
     BKMA_STRATEGY_COLUMN_STREAM = 1, /* lines strided by n2 (n2 >= 64), small shifts: register ring per wavefront */
     BKMA_STRATEGY_LINE_LDS = 2       /* any n2 and shift: slab segments + halos staged in LDS */
@@ -192,7 +196,8 @@ typedef struct bkma_tuned_config {
     int32_t persistent;   /* 1: grid capped at the resident work-groups, waves loop over units */
     int32_t seg_rows;     /* LINE_LDS: rows per segment (0 = whole line in LDS) */
     int32_t wide_window;  /* COLUMN_STREAM, F32 storage: 1 = FP64 register window (rows widened once) */
-    int32_t _reserved[3];
+    int32_t steps_per_pass; /* COLUMN_STREAM: time steps fused into one pass over HBM (0 or 1: one) */
+    int32_t _reserved[2];
 } bkma_tuned_config;
 
 typedef struct bkma_adv_plan bkma_adv_plan;

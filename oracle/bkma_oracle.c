@@ -24,6 +24,8 @@
  * MI355X-specific kernels:
  *   - orc_advect_f32_mixed: FP32 storage, FP64 foot/index/coefficients and
  *     FP64 fused accumulation (the arithmetic of the tuned FP32 kernel);
+ *   - orc_advect_f32_blocked: the same with several steps per pass kept in
+ *     FP64 and one rounding to FP32 per pass;
  *   - orc_conv_torch_f64:   textbook Conv1d (torch.nn.functional.conv1d
  *     semantics), the contraction the MFMA kernels implement.
  *
@@ -219,6 +221,58 @@ static void adv_step_f32_mixed(float *data, const orc_adv_params *p) {
 
 void orc_advect_f32_mixed(float *data, const orc_adv_params *p, uint64_t nsteps) {
     for (uint64_t t = 0; t < nsteps; ++t) adv_step_f32_mixed(data, p);
+}
+
+/* One FP64 step of a single line with the FP32-storage arithmetic above
+ * (fused accumulation), out-of-place: dst[i1] from src. */
+static void adv_line_fma_f64(const orc_adv_params *p, int i0, const double *src, double *dst) {
+    for (int64_t i1 = 0; i1 < (int64_t)p->n1; ++i1) {
+        double d_prev1;
+        const int leftNode = orc_adv_foot(p, i0, (int)i1, &d_prev1);
+        double coef[6];
+        orc_lag_basis(d_prev1, coef);
+        const int ipos1 = leftNode - 2;
+        double acc = 0.;
+        for (int k = 0; k <= 5; k++) {
+            const uint64_t id1 = (p->n1 + (uint64_t)(int64_t)ipos1 + (uint64_t)k) % p->n1;
+            acc = fma(coef[k], src[id1], acc);
+        }
+        dst[i1] = acc;
+    }
+}
+
+/* FP32 storage with `ns` steps per pass: each pass widens the line to FP64,
+ * makes its steps with the fused FP64 arithmetic of orc_advect_f32_mixed
+ * without rounding in between, and rounds to FP32 once.  nsteps / ns full
+ * passes, then one pass of nsteps % ns steps.  ns = 1 is
+ * orc_advect_f32_mixed. */
+void orc_advect_f32_blocked(float *data, const orc_adv_params *p, uint64_t nsteps, uint64_t ns) {
+    if (ns == 0) ns = 1;
+    const int64_t n0 = (int64_t)p->n0, n1 = (int64_t)p->n1, n2 = (int64_t)p->n2;
+#pragma omp parallel
+    {
+        double *a = (double *)malloc(sizeof(double) * (size_t)n1);
+        double *b = (double *)malloc(sizeof(double) * (size_t)n1);
+#pragma omp for schedule(static)
+        for (int64_t line = 0; line < n0 * n2; ++line) {
+            const int64_t i0 = line / n2, i2 = line % n2;
+            float *base = data + (i0 * n1) * n2 + i2;
+            for (uint64_t done = 0; done < nsteps;) {
+                const uint64_t k = nsteps - done < ns ? nsteps - done : ns;
+                for (int64_t i1 = 0; i1 < n1; ++i1) a[i1] = (double)base[i1 * n2];
+                for (uint64_t s = 0; s < k; ++s) {
+                    adv_line_fma_f64(p, (int)i0, a, b);
+                    double *t = a;
+                    a = b;
+                    b = t;
+                }
+                for (int64_t i1 = 0; i1 < n1; ++i1) base[i1 * n2] = (float)a[i1];
+                done += k;
+            }
+        }
+        free(a);
+        free(b);
+    }
 }
 
 /* FP32 storage through the plain executors: the reference's FP64

@@ -61,6 +61,7 @@ def lib():
         L.orc_advect_f64.argtypes = [dp, P(AdvParams), u64]
         L.orc_advect_f32_mixed.argtypes = [fp, P(AdvParams), u64]
         L.orc_advect_f32_plain.argtypes = [fp, P(AdvParams), u64]
+        L.orc_advect_f32_blocked.argtypes = [fp, P(AdvParams), u64, u64]
         L.orc_fill_adv_sin4.argtypes = [dp, P(AdvParams)]
         L.orc_l1_error_adv.restype = ctypes.c_double
         L.orc_l1_error_adv.argtypes = [dp, P(AdvParams), dp]
@@ -121,6 +122,10 @@ def advect_f64(data: np.ndarray, p: AdvParams, nsteps: int) -> None:
 
 def advect_f32_mixed(data: np.ndarray, p: AdvParams, nsteps: int) -> None:
     lib().orc_advect_f32_mixed(_fptr(data), ctypes.byref(p), nsteps)
+
+
+def advect_f32_blocked(data: np.ndarray, p: AdvParams, nsteps: int, steps_per_pass: int) -> None:
+    lib().orc_advect_f32_blocked(_fptr(data), ctypes.byref(p), nsteps, steps_per_pass)
 
 
 def advect_f32_plain(data: np.ndarray, p: AdvParams, nsteps: int) -> None:

@@ -99,7 +99,8 @@ class TunedConfig(ctypes.Structure):
     _fields_ = [("strategy", ctypes.c_int32), ("vec", ctypes.c_int32), ("smax", ctypes.c_int32),
                 ("prefetch", ctypes.c_int32), ("waves_per_wg", ctypes.c_int32),
                 ("nontemporal", ctypes.c_int32), ("persistent", ctypes.c_int32), ("seg_rows", ctypes.c_int32),
-                ("wide_window", ctypes.c_int32), ("_reserved", ctypes.c_int32 * 3)]
+                ("wide_window", ctypes.c_int32), ("steps_per_pass", ctypes.c_int32),
+                ("_reserved", ctypes.c_int32 * 2)]
 
 
 STRATEGY_COLUMN_STREAM = 1

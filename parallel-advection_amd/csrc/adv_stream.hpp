@@ -1,4 +1,33 @@
-// Device side of the column-stream advection kernel (see bkma_tuned.hip).
+// Column-stream advection kernel: device code, launchers and the variant
+// table type (the tables are instantiated in stream_f32.hip / stream_f64.hip,
+// the plans in bkma_tuned.hip).
+//
+// One wavefront owns 64*VEC consecutive i2 columns of one i0 slab and walks
+// the line in increasing i1 ("positions").  Each lane keeps, in registers, a
+// window of WS = 2*SMAX+6 consecutive rows per stage: stage 0 holds rows
+// read from memory, stage t (1 <= t < NS) holds the rows after t time steps
+// (FP64), and stage NS is written back.  NS time steps are thus fused into
+// one pass over HBM (one read and one write per cell for NS steps).
+//
+// Periodic wrap and in-place safety.  Positions run 0 .. R+n1+H-1 with
+// R = NS*BACK and H = NS*AHEAD: positions [0, R) are rows n1-R .. n1-1,
+// positions [R, R+n1) are rows 0 .. n1-1 and positions [R+n1, R+n1+H) are
+// rows 0 .. H-1 again.  Stage t at position p produces the value of the
+// row at p - t*AHEAD after t steps; only positions [R, R+n1) of the last
+// stage are stored, each once, and every row is read from HBM before it is
+// written.  Rows 0 .. H-1 are written before their second use, so their
+// original values are kept in LDS.
+//
+// Foot location (displ, floor, d_prev1) and lag_basis are FP64 and in the
+// reference's operation order.  They depend only on (i0, i1); each trip of
+// U positions computes them once per position, one position per lane, and
+// broadcasts them with readlane (identical bits to a per-cell evaluation).
+// FP64 storage accumulates with separately rounded multiply and add as the
+// reference does, and every intermediate stage holds exactly the FP64 value
+// the reference stores after that step, so NS-step passes are bit-identical
+// to NS single steps.  FP32 storage accumulates in FP64 with fused
+// multiply-adds, keeps intermediate steps in FP64 and rounds to FP32 once
+// per pass.
 #pragma once
 
 #include <type_traits>
@@ -16,6 +45,7 @@ template <int B, int E, class F> __device__ __forceinline__ void static_for(F &&
 
 constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
 constexpr int clcm(int a, int b) { return a / cgcd(a, b) * b; }
+constexpr int cmod(int a, int m) { return ((a % m) + m) % m; }
 
 // Native vector types for 4/8/16-byte global accesses.
 template <class T, int VEC> struct NativeVec { using type = T __attribute__((ext_vector_type(VEC))); };
@@ -57,20 +87,34 @@ __device__ __forceinline__ double wave_uniform(double x) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-// Shape of the stencil window for shifts d = i1 - leftNode in [-SMAX, SMAX].
-template <int SMAX, int P> struct StreamShape {
+// Lane L's value (L a compile-time lane index), as a wave-uniform scalar.
+template <int L> __device__ __forceinline__ int lane_value(int x) { return __builtin_amdgcn_readlane(x, L); }
+template <int L> __device__ __forceinline__ double lane_value(double x) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, L);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), L);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Shape of the per-stage window for shifts d = i1 - leftNode in [-SMAX, SMAX]:
+// the stencil of the row at position q spans positions q-d-2 .. q-d+3.
+template <int SMAX, int P, int NS> struct StreamShape {
     static constexpr int DLO = -SMAX, DHI = SMAX;
-    static constexpr int BACK = SMAX + 2;   // rows behind the output row
-    static constexpr int AHEAD = SMAX + 3;  // rows in front of it
+    static constexpr int BACK = SMAX + 2;   // positions behind the output position
+    static constexpr int AHEAD = SMAX + 3;  // positions in front of it (stage lag)
     static constexpr int WS = BACK + AHEAD + 1;
-    static constexpr int U = clcm(WS, P);   // unrolled rows per loop trip
+    static constexpr int U = clcm(WS, P);  // unrolled positions per loop trip
+    static constexpr int R = NS * BACK;    // positions before row 0
+    static constexpr int H = NS * AHEAD;   // rows re-read from LDS at the end
+    static constexpr int TB = NS * AHEAD;  // foot-table lane of offset 0
+    static_assert(U + TB <= 64, "foot table must fit one wavefront");
 };
 
-// 6-tap interpolation of VEC cells from the window slots S0..S0+5 (mod WS),
-// in FP64.  FP64 storage: the reference's separately rounded multiply and
-// add; FP32 storage: fused FP64 multiply-add.
-template <class T, class WT, int VEC, int WS, int S0>
-__device__ __forceinline__ void interp_window(const WT (&win)[WS][VEC], const double (&c)[6], T (&out)[VEC]) {
+// 6-tap interpolation of VEC cells from window slots S0..S0+5 (mod WS), in
+// FP64.  FP64 storage (sizeof(T) == 8): separately rounded multiply and add;
+// FP32 storage: fused multiply-add.
+template <class T, class WT, class OT, int VEC, int WS, int S0>
+__device__ __forceinline__ void interp_window(const WT (&win)[WS][VEC], const double (&c)[6], OT (&out)[VEC]) {
 #pragma unroll
     for (int v = 0; v < VEC; ++v) {
         double acc = 0.;
@@ -81,23 +125,23 @@ __device__ __forceinline__ void interp_window(const WT (&win)[WS][VEC], const do
 #pragma unroll
             for (int k = 0; k < 6; ++k) acc = fma(c[k], (double)win[(S0 + k) % WS][v], acc);
         }
-        out[v] = (T)acc;
+        out[v] = (OT)acc;
     }
 }
 
-// Select the window slots for the (wave-uniform) shift d at loop position J.
-template <class T, class WT, int VEC, int WS, int J, int BACK, int D, int DHI>
+// Interpolation of the value at position offset OFF (mod WS) for the
+// wave-uniform shift d: input slots OFF-d-2 .. OFF-d+3.
+template <class T, class WT, class OT, int VEC, int WS, int OFF, int D, int DHI>
 __device__ __forceinline__ void interp_shift(int d, const WT (&win)[WS][VEC], const double (&c)[6],
-                                             T (&out)[VEC]) {
+                                             OT (&out)[VEC]) {
     if constexpr (D > DHI) {
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) out[v] = (T)NAN;  // outside the planned window: poison, never read memory
+        for (int v = 0; v < VEC; ++v) out[v] = (OT)NAN;  // outside the planned window: poison, never read memory
     } else {
         if (d == D) {
-            constexpr int S0 = (((J + BACK - D - 2) % WS) + WS) % WS;
-            interp_window<T, WT, VEC, WS, S0>(win, c, out);
+            interp_window<T, WT, OT, VEC, WS, cmod(OFF - D - 2, WS)>(win, c, out);
         } else {
-            interp_shift<T, WT, VEC, WS, J, BACK, D + 1, DHI>(d, win, c, out);
+            interp_shift<T, WT, OT, VEC, WS, OFF, D + 1, DHI>(d, win, c, out);
         }
     }
 }
@@ -109,16 +153,15 @@ struct StreamArgs {
     i64 wave_stride;     // waves in the grid (persistent loop stride)
 };
 
-// One wavefront streams unit (i0, column tile) through the whole line.
-// Row r of the line lives, in turn, in: the staging ring (loaded, P rows
-// ahead), the register window (rows q-BACK .. q+AHEAD around output q).
-// WT: window element type -- double (FP32 rows widened once, on entry) or
-// the storage type (widened at each use; fewer registers).
-template <class T, class WT, int VEC, int SMAX, int P, bool NT>
+// One wavefront advances unit (i0, column tile) by NS steps.
+// WT: stage-0 window element type -- the storage type (widened at each use)
+// or double (widened once, on entry).
+template <class T, class WT, int VEC, int SMAX, int P, int NS, bool NT>
 __device__ __forceinline__ void stream_unit(T *__restrict__ data, const StreamArgs &a, i64 unit, int lane,
                                             T (*head)[64 * VEC]) {
-    using S = StreamShape<SMAX, P>;
-    constexpr int BACK = S::BACK, AHEAD = S::AHEAD, WS = S::WS, U = S::U;
+    using S = StreamShape<SMAX, P, NS>;
+    constexpr int AHEAD = S::AHEAD, WS = S::WS, U = S::U, R = S::R, H = S::H, TB = S::TB;
+    constexpr int NM = NS > 1 ? NS - 1 : 1;
     using M = VecMem<T, VEC, NT>;
     const AdvDev &p = a.p;
     const i64 n1 = p.n1, n2 = p.n2;
@@ -127,10 +170,23 @@ __device__ __forceinline__ void stream_unit(T *__restrict__ data, const StreamAr
     const i64 col = (tile * 64 + lane) * VEC;
     const bool lane_ok = col < n2;  // VEC divides n2: the whole vector is in range
     T *slab = data + (i0 * n1) * n2 + (lane_ok ? col : 0);
+    const i64 first_end = R + n1;  // end of the positions read from HBM
+    const i64 total = first_end + H;
 
-    WT win[WS][VEC];
+    WT w0[WS][VEC];
+    double wm[NM][WS][VEC];
     T stg[P][VEC];
-    auto load_row = [&](i64 row, T (&dst)[VEC]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int s = 0; s < WS; ++s)
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            w0[s][v] = (WT)0;
+#pragma unroll
+            for (int t = 0; t < NM; ++t) wm[t][s][v] = 0.;
+        }
+
+    auto load_pos = [&](i64 pos, T (&dst)[VEC]) __attribute__((always_inline)) {
+        const i64 row = pos < R ? pos - R + n1 : pos - R;  // pos < first_end
         if (lane_ok) {
             M::load(slab + row * n2, dst);
         } else {
@@ -138,90 +194,146 @@ __device__ __forceinline__ void stream_unit(T *__restrict__ data, const StreamAr
             for (int v = 0; v < VEC; ++v) dst[v] = (T)0;
         }
     };
-
-    // Prologue: window rows [-BACK, AHEAD) (tail rows wrap to n1-BACK..n1-1),
-    // rows [0, AHEAD) also kept for the wrap at the end of the line; staging
-    // rows [AHEAD, AHEAD+P).
-    static_for<0, BACK + AHEAD>([&](auto sc) __attribute__((always_inline)) {
-        constexpr int s = decltype(sc)::value;
-        constexpr int r = s - BACK;
-        T tmp[VEC];
-        load_row(r < 0 ? n1 + r : r, tmp);
-#pragma unroll
-        for (int v = 0; v < VEC; ++v) win[s][v] = (WT)tmp[v];
-        if constexpr (r >= 0) {
-#pragma unroll
-            for (int v = 0; v < VEC; ++v) head[r][lane * VEC + v] = tmp[v];
-        }
-    });
     static_for<0, P>([&](auto sc) __attribute__((always_inline)) {
         constexpr int s = decltype(sc)::value;
-        load_row(AHEAD + s, stg[(AHEAD + s) % P]);
+        load_pos(s, stg[s]);
     });
 
-    double coef[6];
-    unsigned long long cached_bits = ~0ull;  // not the bits of any d_prev1 >= 0
     const int half = (int)(n1 / 2);
-    i64 base = 0;
+    int tab_d = 0;
+    double tab_c[6];
 
-    auto row_step = [&](auto jc) __attribute__((always_inline)) {
-        constexpr int J = decltype(jc)::value;
-        const i64 q = base + J;
-        if (q >= n1) return;
-        // 1. row q+AHEAD enters the window (slot of row q-BACK-1)
+    for (i64 base = 0; base < total; base += U) {
+        // Foot table: lane l describes the row at position base - TB + l.
         {
-            constexpr int sw = (J + AHEAD + BACK) % WS;
-            constexpr int ss = (J + AHEAD) % P;
-            const i64 ra = q + AHEAD;
-            if (ra < n1) {
-#pragma unroll
-                for (int v = 0; v < VEC; ++v) win[sw][v] = (WT)stg[ss][v];
-                // 2. refill that staging slot with row q+AHEAD+P
-                const i64 rn = ra + P;
-                if (rn < n1) load_row(rn, stg[ss]);
-            } else {
-                const int h = (int)(ra - n1);  // < AHEAD: the line's first rows, saved before any write
-#pragma unroll
-                for (int hh = 0; hh < AHEAD; ++hh)
-                    if (h == hh) {
-#pragma unroll
-                        for (int v = 0; v < VEC; ++v) win[sw][v] = (WT)head[hh][lane * VEC + v];
-                    }
-            }
+            i64 row = base - TB + lane - R;
+            if (row < 0) row += n1;
+            if (row >= n1) row -= n1;
+            if (row < 0 || row >= n1) row = 0;  // positions no stored value depends on
+            double d_prev1;
+            const int leftNode = adv_foot(p, (int)i0, (int)row, d_prev1);
+            adv_lag_basis(d_prev1, tab_c);
+            int d = (int)row - leftNode;
+            if (d > half) d -= (int)n1;
+            if (d < -half) d += (int)n1;
+            tab_d = d;
         }
-        // 3. FP64 foot location of row q, identical in every lane
-        double d_prev1;
-        const int leftNode = adv_foot(p, (int)i0, (int)q, d_prev1);
-        const unsigned long long bits = __builtin_bit_cast(unsigned long long, wave_uniform(d_prev1));
-        if (bits != cached_bits) {
-            double c[6];
-            adv_lag_basis(__builtin_bit_cast(double, bits), c);
-#pragma unroll
-            for (int k = 0; k < 6; ++k) coef[k] = wave_uniform(c[k]);
-            cached_bits = bits;
-        }
-        int d = wave_uniform((int)q - leftNode);
-        if (d > half) d -= (int)n1;
-        if (d < -half) d += (int)n1;
-        // 4. stencil rows q-d-2 .. q-d+3 from the window, 5. store row q
-        T out[VEC];
-        interp_shift<T, WT, VEC, WS, J, BACK, S::DLO, S::DHI>(d, win, coef, out);
-        if (lane_ok) M::store(slab + q * n2, out);
-    };
 
-    for (; base < n1; base += U) static_for<0, U>(row_step);
+        auto step = [&](auto jc) __attribute__((always_inline)) {
+            constexpr int J = decltype(jc)::value;
+            const i64 pin = base + J;
+            if (pin >= total) return;
+            // 1. position pin enters the stage-0 window
+            {
+                constexpr int sw = J % WS, ss = J % P;
+                if (pin < first_end) {
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) w0[sw][v] = (WT)stg[ss][v];
+                    if (pin >= R && pin < R + H) {
+#pragma unroll
+                        for (int v = 0; v < VEC; ++v) head[pin - R][lane * VEC + v] = stg[ss][v];
+                    }
+                    // 2. refill the staging slot with position pin + P
+                    if (pin + P < first_end) load_pos(pin + P, stg[ss]);
+                } else {
+                    const int h = (int)(pin - first_end);
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) w0[sw][v] = (WT)head[h][lane * VEC + v];
+                }
+            }
+            // 3. stage t computes position pin - t*AHEAD from stage t-1
+            static_for<1, NS + 1>([&](auto tc) __attribute__((always_inline)) {
+                constexpr int t = decltype(tc)::value;
+                constexpr int OFF = J - t * AHEAD;
+                constexpr int L = OFF + TB;
+                const int d = lane_value<L>(tab_d);
+                double c[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) c[k] = lane_value<L>(tab_c[k]);
+                if constexpr (t < NS) {
+                    double o[VEC];
+                    if constexpr (t == 1)
+                        interp_shift<T, WT, double, VEC, WS, OFF, S::DLO, S::DHI>(d, w0, c, o);
+                    else
+                        interp_shift<T, double, double, VEC, WS, OFF, S::DLO, S::DHI>(d, wm[t - 2], c, o);
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) wm[t - 1][cmod(OFF, WS)][v] = o[v];
+                } else {
+                    T o[VEC];
+                    if constexpr (NS == 1)
+                        interp_shift<T, WT, T, VEC, WS, OFF, S::DLO, S::DHI>(d, w0, c, o);
+                    else
+                        interp_shift<T, double, T, VEC, WS, OFF, S::DLO, S::DHI>(d, wm[NS - 2], c, o);
+                    const i64 pt = pin - NS * AHEAD;
+                    if (pt >= R && pt < first_end && lane_ok) M::store(slab + (pt - R) * n2, o);
+                }
+            });
+        };
+        static_for<0, U>(step);
+    }
 }
 
-template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT>
+template <class T, class WT, int VEC, int SMAX, int P, int NS, int WPB, bool NT>
 __global__ __launch_bounds__(64 * WPB) void adv_column_stream_kernel(T *__restrict__ data, StreamArgs a) {
-    using S = StreamShape<SMAX, P>;
-    __shared__ T head_lds[WPB][S::AHEAD][64 * VEC];
+    using S = StreamShape<SMAX, P, NS>;
+    __shared__ T head_lds[WPB][S::H][64 * VEC];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     // Persistent waves: unit = wave, wave + stride, ...  (all lanes of a wave
     // take the same trip count; there is no work-group barrier).
     for (i64 unit = (i64)blockIdx.x * WPB + wv; unit < a.total_units; unit += a.wave_stride)
-        stream_unit<T, WT, VEC, SMAX, P, NT>(data, a, unit, lane, head_lds[wv]);
+        stream_unit<T, WT, VEC, SMAX, P, NS, NT>(data, a, unit, lane, head_lds[wv]);
 }
+
+// ---------------------------------------------------------------------------
+// Host side: launchers for steps-per-pass 1 .. NSMAX of one variant
+// ---------------------------------------------------------------------------
+using StreamLaunchFn = void (*)(void *data, const StreamArgs &a, unsigned grid, hipStream_t s, int ns);
+using StreamOccFn = int (*)(int ns);
+
+template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, int NS>
+void launch_stream(void *data, const StreamArgs &a, unsigned grid, hipStream_t s, int ns) {
+    if constexpr (NS >= 1) {
+        if (ns == NS) {
+            hipLaunchKernelGGL((adv_column_stream_kernel<T, WT, VEC, SMAX, P, NS, WPB, NT>), dim3(grid),
+                               dim3(64 * WPB), 0, s, reinterpret_cast<T *>(data), a);
+            return;
+        }
+        launch_stream<T, WT, VEC, SMAX, P, WPB, NT, NS - 1>(data, a, grid, s, ns);
+    }
+}
+
+// Resident work-groups per CU (0 if the query fails or ns is not compiled).
+template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, int NS> int occupancy_stream(int ns) {
+    if constexpr (NS >= 1) {
+        if (ns == NS) {
+            int blocks = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &blocks, adv_column_stream_kernel<T, WT, VEC, SMAX, P, NS, WPB, NT>, 64 * WPB, 0) != hipSuccess)
+                return 0;
+            return blocks;
+        }
+        return occupancy_stream<T, WT, VEC, SMAX, P, WPB, NT, NS - 1>(ns);
+    }
+    return 0;
+}
+
+struct StreamVariant {
+    bkma_dtype dtype;
+    int vec, smax, prefetch, wpb, nt, wide;  // wide: FP64 stage-0 window for FP32 storage
+    int nsmax;                               // steps per pass compiled: 1 .. nsmax
+    StreamLaunchFn fn;
+    StreamOccFn occ;
+};
+
+#define BKMA_STREAM_VARIANT(T, WT, DT, V, S, P, W, NT, NSMAX)                                               \
+    StreamVariant {                                                                                        \
+        DT, V, S, P, W, NT, sizeof(WT) > sizeof(T), NSMAX, &launch_stream<T, WT, V, S, P, W, NT, NSMAX>,   \
+            &occupancy_stream<T, WT, V, S, P, W, NT, NSMAX>                                                \
+    }
+
+// The compiled variants of each storage type (host functions).
+const StreamVariant *stream_variants_f32(int *count);
+const StreamVariant *stream_variants_f64(int *count);
 
 }  // namespace bkma

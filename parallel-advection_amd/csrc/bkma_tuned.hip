@@ -1,22 +1,16 @@
-// Tuned in-place advection for gfx950.
+// Tuned in-place advection for gfx950: plans, autotuner and C ABI.
 //
 // Strategy COLUMN_STREAM (lines strided by n2 >= 64): one wavefront owns
 // 64*VEC consecutive i2 columns of one i0 slab and walks the whole line in
-// increasing i1.  Each lane keeps a ring of R old rows in registers:
-// loads run AHEAD+P rows in front of the row being written, so every old
-// value a stencil needs is read from HBM before its row is overwritten --
-// the update is in place with exactly one read and one write per cell.
-// The periodic wrap needs the first AHEAD old rows again at the end of the
-// line; they are kept in LDS.  Tail rows (n1-BACK .. n1-1) are read first,
-// before anything is written.
+// increasing i1 with a register window per fused time step (adv_stream.hpp);
+// a plan runs `steps_per_pass` steps per pass over HBM.
+// Strategy LINE_LDS (any n2, any shift): slabs or segments of them staged in
+// LDS (adv_lds.hpp), one step per pass.
 //
-// The FP64 foot location (displ, floor, d_prev1) is computed per row, in
-// the reference's operation order, by every lane of the wave (all lanes of
-// a wave share i0 and i1).  lag_basis is re-evaluated only when d_prev1's
-// bits change, which keeps the coefficients bit-identical to a per-cell
-// evaluation.  FP64 storage accumulates with separately rounded multiply
-// and add as the reference does; FP32 storage accumulates in FP64 with
-// fused multiply-adds and rounds once to FP32 at the store.
+// FP64 storage: results bit-identical to the reference's FP64 steps.
+// FP32 storage: FP64 foot location / index / lag_basis exactly as the
+// reference, FP64 fused accumulation, FP64 intermediate steps inside a pass
+// and one rounding to FP32 per pass.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -35,53 +29,21 @@ namespace bkma {
 // ---------------------------------------------------------------------------
 namespace {
 
-using LaunchFn = void (*)(void *data, const StreamArgs &a, unsigned grid, hipStream_t s);
-using OccFn = int (*)();
-
-template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT>
-void launch_stream(void *data, const StreamArgs &a, unsigned grid, hipStream_t s) {
-    hipLaunchKernelGGL((adv_column_stream_kernel<T, WT, VEC, SMAX, P, WPB, NT>), dim3(grid), dim3(64 * WPB), 0, s,
-                       reinterpret_cast<T *>(data), a);
+// The compiled column-stream variants of a storage type.
+struct VariantSpan {
+    const StreamVariant *b;
+    int n;
+    const StreamVariant *begin() const { return b; }
+    const StreamVariant *end() const { return b + n; }
+};
+VariantSpan stream_variants(bkma_dtype dt) {
+    VariantSpan v{nullptr, 0};
+    if (dt == BKMA_F64) v.b = stream_variants_f64(&v.n);
+    if (dt == BKMA_F32) v.b = stream_variants_f32(&v.n);
+    return v;
 }
 
-// Resident work-groups per CU for a variant (0 if the query fails).
-template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT> int occupancy_stream() {
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, adv_column_stream_kernel<T, WT, VEC, SMAX, P, WPB, NT>,
-                                                     64 * WPB, 0) != hipSuccess)
-        return 0;
-    return blocks;
-}
-
-struct StreamVariant {
-    bkma_dtype dtype;
-    int vec, smax, prefetch, wpb, nt, wide;  // wide: FP64 window for FP32 storage
-    LaunchFn fn;
-    OccFn occ;
-};
-
-#define SV(T, WT, DT, V, S, P, W, NT)                                                                      \
-    StreamVariant{DT, V, S, P, W, NT, sizeof(WT) > sizeof(T), &launch_stream<T, WT, V, S, P, W, NT>,       \
-                  &occupancy_stream<T, WT, V, S, P, W, NT>}
-// P is WS/2 or WS (WS = 2*SMAX + 6) so the unrolled loop stays one window long.
-const StreamVariant kStreamVariants[] = {
-#ifdef BKMA_TUNED_FEW
-    SV(float, float, BKMA_F32, 4, 2, 10, 1, true),
-#else
-    SV(float, double, BKMA_F32, 4, 2, 5, 1, true),  SV(float, double, BKMA_F32, 4, 2, 10, 1, true),
-    SV(float, float, BKMA_F32, 4, 2, 5, 1, true),   SV(float, float, BKMA_F32, 4, 2, 10, 1, true),
-    SV(float, float, BKMA_F32, 4, 2, 10, 1, false), SV(float, double, BKMA_F32, 2, 2, 10, 1, true),
-    SV(float, float, BKMA_F32, 2, 2, 10, 1, true),  SV(float, double, BKMA_F32, 1, 2, 10, 1, false),
-    SV(float, float, BKMA_F32, 4, 1, 8, 1, true),   SV(float, double, BKMA_F32, 4, 1, 8, 1, true),
-    SV(float, float, BKMA_F32, 2, 1, 8, 1, true),   SV(float, float, BKMA_F32, 4, 4, 7, 1, true),
-    SV(float, double, BKMA_F32, 2, 4, 14, 1, true), SV(float, double, BKMA_F32, 1, 4, 14, 1, false),
-    SV(double, double, BKMA_F64, 2, 2, 5, 1, true), SV(double, double, BKMA_F64, 2, 2, 10, 1, true),
-    SV(double, double, BKMA_F64, 2, 2, 5, 1, false), SV(double, double, BKMA_F64, 1, 2, 10, 1, false),
-    SV(double, double, BKMA_F64, 2, 1, 8, 1, true), SV(double, double, BKMA_F64, 2, 4, 7, 1, true),
-    SV(double, double, BKMA_F64, 1, 4, 14, 1, false),
-#endif
-};
-#undef SV
+int steps_per_pass(const bkma_tuned_config &c) { return c.steps_per_pass > 1 ? c.steps_per_pass : 1; }
 
 // Range of the backward shift d = i1 - leftNode over the velocity grid.
 // leftNode = floor(i1 - s + e) with s = v*dt/dx and |e| << 1 (rounding), so
@@ -107,7 +69,8 @@ struct TunedPlan {
     std::string name;
     // COLUMN_STREAM
     const StreamVariant *variant = nullptr;
-    i64 persistent_blocks = 0;  // 0: one wave per unit; else grid cap (resident work-groups)
+    int ns = 1;                        // steps per pass
+    i64 persistent_blocks[9] = {};    // by steps per pass: 0 = one wave per unit, else the grid cap
     // LINE_LDS
     LdsArgs lds{};
     u64 lds_bytes = 0;
@@ -145,8 +108,13 @@ bool shift_supported(const bkma_adv_params &p, std::string *why) {
     return true;
 }
 
-bkma_status stream_feasible(const bkma_adv_params &p, bkma_dtype dt, const StreamVariant &v, std::string *why) {
+bkma_status stream_feasible(const bkma_adv_params &p, bkma_dtype dt, const StreamVariant &v, int ns,
+                            std::string *why) {
     if (v.dtype != dt) return BKMA_ERR_UNSUPPORTED;
+    if (ns < 1 || ns > v.nsmax) {
+        if (why) *why = "steps per pass not compiled for this variant";
+        return BKMA_ERR_UNSUPPORTED;
+    }
     if (!shift_supported(p, why)) return BKMA_ERR_UNSUPPORTED;
     if (p.n2 % (u64)v.vec != 0) {
         if (why) *why = "n2 not a multiple of VEC";
@@ -159,7 +127,7 @@ bkma_status stream_feasible(const bkma_adv_params &p, bkma_dtype dt, const Strea
         return BKMA_ERR_UNSUPPORTED;
     }
     const int WS = 2 * v.smax + 6;
-    if (p.n1 < (u64)(2 * (WS + v.prefetch) + 8)) {
+    if (p.n1 < (u64)(ns * (WS - 1) + v.prefetch + WS + 8)) {
         if (why) *why = "line shorter than the register ring";
         return BKMA_ERR_UNSUPPORTED;
     }
@@ -216,28 +184,38 @@ bkma_status launch_lds(const TunedPlan &pl, void *data, hipStream_t s) {
     return check_launch("adv_line_lds_kernel");
 }
 
-bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s) {
+// One pass of ns steps (ns <= the plan's steps per pass).
+bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s, int ns) {
     if (pl.cfg.strategy == BKMA_STRATEGY_LINE_LDS)
         return pl.dtype == BKMA_F64 ? launch_lds<double>(pl, data, s) : launch_lds<float>(pl, data, s);
+    if (ns < 1 || ns > pl.variant->nsmax) return fail(BKMA_ERR_RUNTIME, "tuned advection: steps per pass out of range");
     const StreamVariant &v = *pl.variant;
     StreamArgs a;
     a.p = make_adv_dev(pl.params);
     a.tiles_per_slab = (i64)ceil_div(pl.params.n2, (u64)(64 * v.vec));
     a.total_units = (i64)pl.params.n0 * a.tiles_per_slab;
     u64 grid = ceil_div((u64)a.total_units, (u64)v.wpb);
-    if (pl.persistent_blocks > 0) grid = std::min<u64>(grid, (u64)pl.persistent_blocks);
+    if (pl.persistent_blocks[ns] > 0) grid = std::min<u64>(grid, (u64)pl.persistent_blocks[ns]);
     grid = std::min<u64>(grid, 0x7fffffffULL / (u64)(64 * v.wpb));
     if (grid == 0) return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: empty grid");
     a.wave_stride = (i64)grid * v.wpb;
-    v.fn(data, a, (unsigned)grid, s);
+    v.fn(data, a, (unsigned)grid, s, ns);
     return check_launch("adv_column_stream_kernel");
 }
 
-std::string stream_name(const StreamVariant &v, bool persistent) {
+// nsteps steps: full passes of the plan's steps per pass, then the rest.
+bkma_status plan_steps(const TunedPlan &pl, void *data, u64 nsteps, hipStream_t s) {
+    const u64 ns = (u64)pl.ns;
+    for (u64 t = 0; t + ns <= nsteps; t += ns) BKMA_TRY(plan_launch(pl, data, s, (int)ns));
+    if (nsteps % ns) BKMA_TRY(plan_launch(pl, data, s, (int)(nsteps % ns)));
+    return BKMA_OK;
+}
+
+std::string stream_name(const StreamVariant &v, int ns, bool persistent) {
     return std::string("column_stream<") + (v.dtype == BKMA_F64 ? "f64" : (v.wide ? "f32,win64" : "f32,win32")) +
-           ",vec" + std::to_string(v.vec) +
-           ",smax" + std::to_string(v.smax) + ",p" + std::to_string(v.prefetch) + ",wpb" + std::to_string(v.wpb) +
-           (v.nt ? ",nt" : "") + (persistent ? ",persistent" : "") + ">";
+           ",vec" + std::to_string(v.vec) + ",smax" + std::to_string(v.smax) + ",p" + std::to_string(v.prefetch) +
+           ",wpb" + std::to_string(v.wpb) + ",steps" + std::to_string(ns) + (v.nt ? ",nt" : "") +
+           (persistent ? ",persistent" : "") + ">";
 }
 
 std::string lds_name(bkma_dtype dt, const LdsArgs &a) {
@@ -246,11 +224,11 @@ std::string lds_name(bkma_dtype dt, const LdsArgs &a) {
 }
 
 // Resident work-groups on the whole device for a variant (persistent grid).
-i64 resident_blocks(const StreamVariant &v) {
+i64 resident_blocks(const StreamVariant &v, int ns) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    return (i64)v.occ() * cus;
+    return (i64)v.occ(ns) * cus;
 }
 
 // A candidate plan (not yet owning device memory).
@@ -264,20 +242,27 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
     pl->dtype = dt;
     pl->cfg = cfg;
     if (cfg.strategy == BKMA_STRATEGY_COLUMN_STREAM) {
-        for (const auto &v : kStreamVariants) {
-            if (v.dtype == dt && v.vec == cfg.vec && v.smax == cfg.smax && v.prefetch == cfg.prefetch &&
-                v.wpb == cfg.waves_per_wg && v.nt == (cfg.nontemporal != 0) && v.wide == (cfg.wide_window != 0)) {
-                if (stream_feasible(p, dt, v, &why) != BKMA_OK)
+        const int ns = steps_per_pass(cfg);
+        for (const auto &v : stream_variants(dt)) {
+            if (v.vec == cfg.vec && v.smax == cfg.smax && v.prefetch == cfg.prefetch && v.wpb == cfg.waves_per_wg &&
+                v.nt == (cfg.nontemporal != 0) && v.wide == (cfg.wide_window != 0) && ns <= v.nsmax) {
+                if (stream_feasible(p, dt, v, ns, &why) != BKMA_OK)
                     return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
                 pl->variant = &v;
-                pl->persistent_blocks = cfg.persistent ? resident_blocks(v) : 0;
-                pl->name = stream_name(v, pl->persistent_blocks > 0);
+                pl->ns = ns;
+                pl->cfg.steps_per_pass = ns;
+                for (int k = 1; k <= ns; ++k) pl->persistent_blocks[k] = cfg.persistent ? resident_blocks(v, k) : 0;
+                pl->cfg.persistent = pl->persistent_blocks[ns] > 0;
+                pl->name = stream_name(v, ns, pl->persistent_blocks[ns] > 0);
                 return BKMA_OK;
             }
         }
         return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: no compiled variant matches the configuration");
     }
     if (cfg.strategy == BKMA_STRATEGY_LINE_LDS) {
+        if (steps_per_pass(cfg) != 1)
+            return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: LINE_LDS runs one step per pass");
+        pl->cfg.steps_per_pass = 1;
         if (lds_shape(p, dt, cfg.seg_rows, &pl->lds, &pl->lds_bytes, &why) != BKMA_OK)
             return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
         if (pl->lds.nseg > 1) {
@@ -300,22 +285,26 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
 std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype dt, std::string *why) {
     std::vector<bkma_tuned_config> out;
     int smin_needed = 1 << 30;
-    for (const auto &v : kStreamVariants)
-        if (stream_feasible(p, dt, v, why) == BKMA_OK) smin_needed = std::min(smin_needed, v.smax);
-    for (const auto &v : kStreamVariants) {
-        if (stream_feasible(p, dt, v, why) != BKMA_OK || v.smax > smin_needed + 2) continue;
-        for (int pers = 0; pers < 2; ++pers) {
-            bkma_tuned_config c;
-            std::memset(&c, 0, sizeof(c));
-            c.strategy = BKMA_STRATEGY_COLUMN_STREAM;
-            c.vec = v.vec;
-            c.smax = v.smax;
-            c.prefetch = v.prefetch;
-            c.waves_per_wg = v.wpb;
-            c.nontemporal = v.nt;
-            c.wide_window = v.wide;
-            c.persistent = pers;
-            out.push_back(c);
+    for (const auto &v : stream_variants(dt))
+        if (stream_feasible(p, dt, v, 1, why) == BKMA_OK) smin_needed = std::min(smin_needed, v.smax);
+    for (const auto &v : stream_variants(dt)) {
+        if (v.smax > smin_needed + 2) continue;
+        for (int ns = 1; ns <= v.nsmax; ++ns) {
+            if (stream_feasible(p, dt, v, ns, why) != BKMA_OK) continue;
+            for (int pers = 0; pers < 2; ++pers) {
+                bkma_tuned_config c;
+                std::memset(&c, 0, sizeof(c));
+                c.strategy = BKMA_STRATEGY_COLUMN_STREAM;
+                c.vec = v.vec;
+                c.smax = v.smax;
+                c.prefetch = v.prefetch;
+                c.waves_per_wg = v.wpb;
+                c.nontemporal = v.nt;
+                c.wide_window = v.wide;
+                c.steps_per_pass = ns;
+                c.persistent = pers;
+                out.push_back(c);
+            }
         }
     }
     const u64 esz = dt == BKMA_F64 ? 8 : 4;
@@ -334,6 +323,7 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
         bkma_tuned_config c;
         std::memset(&c, 0, sizeof(c));
         c.strategy = BKMA_STRATEGY_LINE_LDS;
+        c.steps_per_pass = 1;
         c.seg_rows = (int32_t)(a.nseg > 1 ? a.seg_rows : 0);
         bool dup = false;
         for (auto &o : out) dup |= o.strategy == c.strategy && o.seg_rows == c.seg_rows;
@@ -342,12 +332,13 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
     return out;
 }
 
-// Heuristic: column stream (smallest window, widest vector, deepest
-// prefetch) when it applies, else LINE_LDS whole line, else the longest
-// segment.
+// Heuristic: column stream (smallest window, most steps per pass, widest
+// vector, deepest prefetch) when it applies, else LINE_LDS whole line, else
+// the longest segment.
 int heuristic_score(const bkma_tuned_config &c) {
     if (c.strategy == BKMA_STRATEGY_COLUMN_STREAM)
-        return 1000000000 - c.smax * 100000 + c.vec * 1000 + c.prefetch * 10 - c.nontemporal - c.persistent;
+        return 1000000000 - c.smax * 1000000 + steps_per_pass(c) * 10000 + c.vec * 1000 + c.prefetch * 10 +
+               c.nontemporal - c.persistent - 100 * c.wide_window;
     return c.seg_rows == 0 ? 500000000 : 100000000 + std::min(c.seg_rows, 1 << 24) / 1024;
 }
 
@@ -355,14 +346,14 @@ bkma_status time_plan(const TunedPlan &pl, void *buf, int reps, hipStream_t s, f
     hipEvent_t e0, e1;
     BKMA_HIP_TRY(hipEventCreate(&e0));
     BKMA_HIP_TRY(hipEventCreate(&e1));
-    bkma_status st = plan_launch(pl, buf, s);  // warm-up
+    bkma_status st = plan_launch(pl, buf, s, pl.ns);  // warm-up
     if (st == BKMA_OK) {
         (void)hipEventRecord(e0, s);
-        for (int r = 0; r < reps && st == BKMA_OK; ++r) st = plan_launch(pl, buf, s);
+        for (int r = 0; r < reps && st == BKMA_OK; ++r) st = plan_launch(pl, buf, s, pl.ns);
         (void)hipEventRecord(e1, s);
         (void)hipEventSynchronize(e1);
         (void)hipEventElapsedTime(ms, e0, e1);
-        *ms /= (float)reps;
+        *ms /= (float)(reps * pl.ns);  // per step
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -469,29 +460,30 @@ extern "C" bkma_status bkma_adv_plan_run(bkma_adv_plan *plan, void *data, uint64
     if (!plan || !data) return fail(BKMA_ERR_INVALID_ARGUMENT, "null argument");
     if (reinterpret_cast<uintptr_t>(data) % 16 != 0)
         return fail(BKMA_ERR_INVALID_ARGUMENT, "tuned advection: data must be 16-byte aligned");
-    for (uint64_t t = 0; t < nsteps; ++t) BKMA_TRY(plan_launch(*plan, data, as_stream(stream)));
-    return BKMA_OK;
+    return plan_steps(*plan, data, nsteps, as_stream(stream));
 }
 
 extern "C" void bkma_adv_plan_destroy(bkma_adv_plan *plan) { delete plan; }
 
 extern "C" int32_t bkma_adv_plan_variants(bkma_dtype dtype, bkma_tuned_config *out, int32_t max_out) {
     int32_t n = 0;
-    for (const auto &v : kStreamVariants) {
-        if (v.dtype != dtype) continue;
-        if (out && n < max_out) {
-            bkma_tuned_config c;
-            std::memset(&c, 0, sizeof(c));
-            c.strategy = BKMA_STRATEGY_COLUMN_STREAM;
-            c.vec = v.vec;
-            c.smax = v.smax;
-            c.prefetch = v.prefetch;
-            c.waves_per_wg = v.wpb;
-            c.nontemporal = v.nt;
-            c.wide_window = v.wide;
-            out[n] = c;
+    for (const auto &v : stream_variants(dtype)) {
+        for (int ns = 1; ns <= v.nsmax; ++ns) {
+            if (out && n < max_out) {
+                bkma_tuned_config c;
+                std::memset(&c, 0, sizeof(c));
+                c.strategy = BKMA_STRATEGY_COLUMN_STREAM;
+                c.vec = v.vec;
+                c.smax = v.smax;
+                c.prefetch = v.prefetch;
+                c.waves_per_wg = v.wpb;
+                c.nontemporal = v.nt;
+                c.wide_window = v.wide;
+                c.steps_per_pass = ns;
+                out[n] = c;
+            }
+            ++n;
         }
-        ++n;
     }
     return n;
 }

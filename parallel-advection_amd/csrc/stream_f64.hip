@@ -1,0 +1,25 @@
+// Compiled column-stream variants for FP64 storage (the tuner's search space).
+// Each entry instantiates steps-per-pass 1 .. NSMAX.
+#include "adv_stream.hpp"
+
+namespace bkma {
+
+// (storage, stage-0 window, dtype, VEC, SMAX, P, waves/WG, non-temporal, NSMAX)
+const StreamVariant *stream_variants_f64(int *count) {
+    static const StreamVariant table[] = {
+#ifdef BKMA_TUNED_FEW
+        BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 2, 5, 1, true, 2),
+#else
+        BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 2, 5, 1, true, 3),
+        BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 2, 10, 1, true, 3),
+        BKMA_STREAM_VARIANT(double, double, BKMA_F64, 1, 2, 10, 1, false, 4),
+        BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 1, 8, 1, true, 3),
+        BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 4, 7, 1, true, 2),
+        BKMA_STREAM_VARIANT(double, double, BKMA_F64, 1, 4, 14, 1, false, 3),
+#endif
+    };
+    *count = (int)(sizeof(table) / sizeof(table[0]));
+    return table;
+}
+
+}  // namespace bkma

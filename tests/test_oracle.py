@@ -13,6 +13,10 @@ Pinning map (oracle function -> checks):
                        validation_unittests.cpp:11-94), linearity
   orc_advect_f32_mixed distance to orc_advect_f64 within the FP32 budget,
                        integer-shift bit-exactness
+  orc_advect_f32_blocked one pass of n steps within one FP32 rounding of
+                       orc_advect_f64 (independent FP64 path), integer-shift
+                       bit-exactness, error shrinking with the pass length,
+                       remainder-pass composition
   orc_l1_error_adv     exactly 0 at t = 0 (ValidateNoIteration)
   orc_conv_literal_f64 constant-input known answer 1 + C*K*1.5*7.3 (the
                        reference's fill_buffer_conv1d values, init.hpp:50-72),
@@ -171,6 +175,63 @@ def test_f32_mixed_close_to_f64():
     O.advect_f32_mixed(f, p, 50)
     err = np.abs(f.astype(np.float64) - d).max() / np.abs(d).max()
     assert 0 < err < 1e-6
+
+
+def test_f32_blocked_single_pass_is_one_rounding_of_f64():
+    # n steps in one pass: FP64 throughout (fused instead of separate
+    # multiply-add), then one rounding -- within one FP32 ulp of the rounded
+    # FP64 path, and equal to it almost everywhere.
+    p = O.make_params(8, 512, 4, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    d = O.uniform(8 * 512 * 4, 11).reshape(8, 512, 4)
+    f = d.astype(np.float32)
+    d = f.astype(np.float64)
+    O.advect_f64(d, p, 6)
+    O.advect_f32_blocked(f, p, 6, 6)
+    ref = d.astype(np.float32)
+    ulp = np.spacing(np.abs(ref))
+    assert np.all(np.abs(f - ref) <= ulp)
+    assert np.mean(f == ref) > 0.999
+
+
+@pytest.mark.parametrize("ns", [1, 2, 3, 5])
+def test_f32_blocked_integer_shift_bit_exact(ns):
+    n0, n1, n2 = 4, 1024, 3
+    p = O.make_params(n0, n1, n2, dt=1 / 512, minRealVx=-1, maxRealVx=1)
+    f = O.uniform(n0 * n1 * n2, 12, np.float32).reshape(n0, n1, n2)
+    g = f.copy()
+    O.advect_f32_blocked(g, p, 7, ns)
+    for i0 in range(n0):
+        s = int(round((-1 + i0 / 2) * 2))
+        assert np.array_equal(g[i0], np.roll(f[i0], 7 * s, axis=0))
+
+
+def test_f32_blocked_error_shrinks_with_pass_length():
+    p = O.make_params(16, 1024, 8, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    d = O.uniform(16 * 1024 * 8, 1).reshape(16, 1024, 8)
+    f0 = d.astype(np.float32)
+    O.advect_f64(d, p, 50)
+    errs = []
+    for ns in (1, 2, 5):
+        f = f0.copy()
+        O.advect_f32_blocked(f, p, 50, ns)
+        errs.append(np.abs(f.astype(np.float64) - d).max() / np.abs(d).max())
+    assert errs[0] > errs[1] > errs[2] > 0
+    assert errs[2] < 1e-6
+
+
+def test_f32_blocked_remainder_pass():
+    p = O.make_params(3, 256, 2, dt=2e-3, minRealVx=-1, maxRealVx=1)
+    f = O.uniform(3 * 256 * 2, 13, np.float32).reshape(3, 256, 2)
+    a, b = f.copy(), f.copy()
+    O.advect_f32_blocked(a, p, 7, 3)
+    for k in (3, 3, 1):
+        O.advect_f32_blocked(b, p, k, k)
+    assert np.array_equal(a, b)
+    c = f.copy()
+    O.advect_f32_blocked(c, p, 4, 1)
+    d = f.copy()
+    O.advect_f32_mixed(d, p, 4)
+    assert np.array_equal(c, d)
 
 
 # ---------------------------------------------------------------------------

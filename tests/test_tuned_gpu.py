@@ -1,10 +1,11 @@
 """GPU parity of the tuned advection kernels (bkma_adv_plan_*).
 
 FP64 storage must be bit-identical to the oracle's FP64 restatement of the
-reference; FP32 storage must be bit-identical to the oracle's restatement of
-the tuned FP32 arithmetic (FP64 foot/index/coefficients, fused FP64
-accumulation, one FP32 rounding per step) and within 1e-6 relative max-norm
-of the FP64 result."""
+reference, whatever the number of steps fused per pass; FP32 storage must be
+bit-identical to the oracle's restatement of the tuned FP32 arithmetic (FP64
+foot/index/coefficients, fused FP64 accumulation, FP64 steps inside a pass,
+one FP32 rounding per pass) and within 1e-6 relative max-norm of the FP64
+result."""
 import numpy as np
 import pytest
 
@@ -20,6 +21,7 @@ def _torch():
 
 
 def _run(host, params, dtype, nsteps, config=None, autotune=False):
+    """Runs a plan; returns (result, steps per pass of the plan)."""
     torch = _torch()
     tdt = torch.float64 if dtype == B.F64 else torch.float32
     d = torch.from_numpy(host).to(dtype=tdt, device="cuda")
@@ -29,7 +31,7 @@ def _run(host, params, dtype, nsteps, config=None, autotune=False):
     plan.run(d, nsteps)
     torch.cuda.synchronize()
     plan.close()
-    return d.cpu().numpy()
+    return d.cpu().numpy(), max(cfg.steps_per_pass, 1)
 
 
 SHAPES = [(4, 1024, 512), (3, 333, 64), (5, 200, 192), (2, 96, 96), (7, 1000, 130)]
@@ -41,7 +43,7 @@ def test_tuned_f64_bit_exact(cuda, shape):
     p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
     po = O.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
     host = O.uniform(n0 * n1 * n2, 1).reshape(shape)
-    got = _run(host, p, B.F64, 9)
+    got, _ = _run(host, p, B.F64, 9)
     ref = host.copy()
     O.advect_f64(ref, po, 9)
     assert np.array_equal(got, ref), np.abs(got - ref).max()
@@ -53,15 +55,15 @@ def test_tuned_f32_bit_exact_vs_restatement(cuda, shape):
     p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
     po = O.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
     host = O.uniform(n0 * n1 * n2, 2, np.float32).reshape(shape)
-    got = _run(host, p, B.F32, 9)
+    got, spp = _run(host, p, B.F32, 9)
     ref = host.copy()
-    O.advect_f32_mixed(ref, po, 9)
+    O.advect_f32_blocked(ref, po, 9, spp)
     assert np.array_equal(got, ref), np.abs(got - ref).max()
 
 
 def _variant_id(c):
     return (f"vec{c.vec}-smax{c.smax}-p{c.prefetch}-wpb{c.waves_per_wg}{'-nt' if c.nontemporal else ''}"
-            f"{'-w64' if c.wide_window else ''}")
+            f"{'-w64' if c.wide_window else ''}-steps{c.steps_per_pass}")
 
 
 F32_VARIANTS = B.plan_variants(B.F32)
@@ -78,9 +80,10 @@ def test_every_f32_variant(cuda, cfg, persistent):
     po = O.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
     host = O.uniform(n0 * n1 * n2, 3, np.float32).reshape(n0, n1, n2)
     cfg.persistent = persistent
-    got = _run(host, p, B.F32, 5, config=cfg)
+    got, spp = _run(host, p, B.F32, 5, config=cfg)
+    assert spp == cfg.steps_per_pass
     ref = host.copy()
-    O.advect_f32_mixed(ref, po, 5)
+    O.advect_f32_blocked(ref, po, 5, spp)
     assert np.array_equal(got, ref)
 
 
@@ -91,7 +94,7 @@ def test_every_f64_variant(cuda, cfg):
     p = B.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
     po = O.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
     host = O.uniform(n0 * n1 * n2, 4).reshape(n0, n1, n2)
-    got = _run(host, p, B.F64, 5, config=cfg)
+    got, _ = _run(host, p, B.F64, 5, config=cfg)
     ref = host.copy()
     O.advect_f64(ref, po, 5)
     assert np.array_equal(got, ref)
@@ -102,7 +105,7 @@ def test_integer_shift_known_answer(cuda):
     n0, n1, n2 = 4, 1024, 128
     p = B.make_params(n0, n1, n2, dt=1 / 512, minRealVx=-1, maxRealVx=1)
     host = O.uniform(n0 * n1 * n2, 5).reshape(n0, n1, n2)
-    got = _run(host, p, B.F64, 1)
+    got, _ = _run(host, p, B.F64, 1)
     for i0 in range(n0):
         s = int(round((-1 + i0 / 2) * 2))
         assert np.array_equal(got[i0], np.roll(host[i0], s, axis=0))
@@ -113,9 +116,9 @@ def test_autotune_picks_a_correct_variant(cuda):
     p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
     po = O.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
     host = O.uniform(n0 * n1 * n2, 6, np.float32).reshape(n0, n1, n2)
-    got = _run(host, p, B.F32, 3, autotune=True)
+    got, spp = _run(host, p, B.F32, 7, autotune=True)
     ref = host.copy()
-    O.advect_f32_mixed(ref, po, 3)
+    O.advect_f32_blocked(ref, po, 7, spp)
     assert np.array_equal(got, ref)
 
 
@@ -136,6 +139,14 @@ def test_f32_error_budget_vs_f64(cuda):
     print(f"FP32 vs FP64 after 50 steps: rel_max {rel_max:.3e} rel_L2 {rel_l2:.3e}", flush=True)
     assert rel_max <= 50 * 2.0 ** -24
     assert rel_l2 <= 3e-7
+
+
+def test_steps_per_pass_beyond_compiled_is_rejected(cuda):
+    p = B.make_params(4, 1024, 512, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    cfg = F32_VARIANTS[0]
+    cfg.steps_per_pass = 64
+    with pytest.raises(B.BkmaUnsupported):
+        B.AdvPlan(p, B.F32, config=cfg)
 
 
 def test_unsupported_shapes_are_reported(cuda):
@@ -171,7 +182,8 @@ def test_line_lds_bit_exact(cuda, case, dtype):
     po = O.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
     npdt = np.float64 if dtype == B.F64 else np.float32
     host = O.uniform(n0 * n1 * n2, 7, npdt).reshape(n0, n1, n2)
-    got = _run(host, p, dtype, 4, config=_lds_cfg(seg))
+    got, spp = _run(host, p, dtype, 4, config=_lds_cfg(seg))
+    assert spp == 1
     ref = host.copy()
     if dtype == B.F64:
         O.advect_f64(ref, po, 4)
@@ -186,7 +198,7 @@ def test_autotune_small_n2_long_line(cuda):
     p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
     po = O.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
     host = O.uniform(n0 * n1 * n2, 8, np.float32).reshape(n0, n1, n2)
-    got = _run(host, p, B.F32, 2, autotune=True)
+    got, spp = _run(host, p, B.F32, 2, autotune=True)
     ref = host.copy()
-    O.advect_f32_mixed(ref, po, 2)
+    O.advect_f32_blocked(ref, po, 2, spp)
     assert np.array_equal(got, ref)
