@@ -4,10 +4,11 @@
 //   j in [0, L-K+1) overwrites x[b][oc][j]; x[b][oc][L-K+1 .. L) is kept.
 //
 // In-place safety: y[b][:, J] overwrites x[b][:, J], which every output
-// channel still needs, so one work-group owns one batch element b and walks
-// its columns in increasing tiles [j0, j0+TN): a tile reads columns
-// [j0, j0+TN+K-1) and writes [j0, j0+TN), so reads always run ahead of
-// writes.
+// channel still needs.  The direct kernels give one work-group one batch
+// element and walk its columns in increasing tiles [j0, j0+TN): a tile reads
+// columns [j0, j0+TN+K-1) and writes [j0, j0+TN), so reads always run ahead
+// of writes.  The MFMA kernels process tiles in any order: the K-1 columns a
+// tile reads beyond its own are snapshotted by a pre-pass (conv_halo_kernel).
 //
 // Kernels
 //  - conv_direct_kernel<T>: FP64 / FP32 FMA reference kernels (FP64 keeps the
@@ -126,24 +127,67 @@ __global__ void conv_pack_weights_kernel(const float *__restrict__ w, int C, int
     }
 }
 
-// LDS tile: Xs[jj][ic] bf16, row stride C + 8 (16-byte pad keeps the rows of
-// a ds_read_b128 on different banks).
-template <class TIO, int NPROD, int KS, int NW>
-__global__ __launch_bounds__(64 * NW) void conv_mfma_kernel(TIO *__restrict__ x, const __bf16 *__restrict__ whi,
-                                                           const __bf16 *__restrict__ wlo, const float *__restrict__ bias,
-                                                           int L, int K) {
-    constexpr int C = 32 * NW;     // channels (in = out)
-    constexpr int TN = 64;         // columns per tile (two 32-column MFMA tiles)
-    constexpr int RS = C + 8;      // LDS row stride (elements)
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    __bf16 *xs_hi = reinterpret_cast<__bf16 *>(lds_raw);
-    const int JW = TN + K - 1;
-    __bf16 *xs_lo = xs_hi + JW * RS;
-    const int Lout = L - K + 1;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    TIO *xb = x + (i64)blockIdx.x * C * L;
+// In place without ordering between tiles: tile t of a batch element writes
+// columns [t*TN, t*TN + TN) and reads K-1 more; those belong to tile t+1, so
+// they are copied first, for every tile, into halo[b][t][ic][K-1].
+template <class TIO>
+__global__ void conv_halo_kernel(const TIO *__restrict__ x, TIO *__restrict__ halo, int C, int L, int K, int TN,
+                                 int tiles, i64 total) {
+    const int H = K - 1;
+    for (i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (i64)gridDim.x * blockDim.x) {
+        const int kk = (int)(idx % H);
+        i64 r = idx / H;
+        const int ic = (int)(r % C);
+        r /= C;
+        const int t = (int)(r % (tiles - 1));
+        const i64 b = r / (tiles - 1);
+        const i64 j = (i64)(t + 1) * TN + kk;
+        halo[idx] = j < L ? x[(b * C + ic) * (i64)L + j] : (TIO)0.0f;
+    }
+}
 
-    // Weight fragments of this wave, resident for the whole kernel.
+template <class T, int N> struct IoVec;
+template <> struct IoVec<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct IoVec<__hip_bfloat16, 4> { typedef unsigned short type __attribute__((ext_vector_type(4))); };
+
+// Persistent implicit-GEMM kernel.  A work-group (C/32 waves) owns all C
+// output channels of a tile of TN columns of one batch element; wave w keeps
+// the W fragments of channels 32w..32w+31 in registers.  Tiles are taken in
+// a grid-stride loop, double-buffered in LDS: the next tile's global loads
+// are issued before the current tile's MFMAs and written to the other LDS
+// buffer after them, one barrier per tile.
+// LDS tile: Xs[jj][ic] bf16, row stride C + 8 (16-byte pad).
+template <class TIO, int NPROD, int C, int K, int TN>
+__global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x, const TIO *__restrict__ halo,
+                                                           const __bf16 *__restrict__ whi,
+                                                           const __bf16 *__restrict__ wlo,
+                                                           const float *__restrict__ bias, int L, int tiles,
+                                                           int total_tiles, int vec_ok) {
+    constexpr int NW = C / 32;
+    constexpr int NT = 64 * NW;
+    constexpr int KS = C * K / 16;
+    constexpr int JW = TN + K - 1;
+    constexpr int RS = C + 8;
+    constexpr int NPART = NPROD == 3 ? 2 : 1;
+    constexpr int BUF = JW * RS;  // bf16 elements of one part of one buffer
+    constexpr int NSUB = TN / 32;
+    constexpr int NQ = TN / 4;                // 4-column groups
+    constexpr int NTASK = (C / 8) * NQ;       // (8-channel group, 4-column group) staging tasks
+    static_assert(NTASK % NT == 0, "staging tasks must divide evenly");
+    constexpr int TPT = NTASK / NT;
+    constexpr int NH = C * (K - 1);
+    constexpr int HPT = (NH + NT - 1) / NT;
+    using V4 = typename IoVec<TIO, 4>::type;
+
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __bf16 *xs = reinterpret_cast<__bf16 *>(lds_raw);  // [2][NPART][JW][RS]
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int Lout = L - K + 1;
+
+    int tile = blockIdx.x;
+    if (tile >= total_tiles) return;  // uniform over the work-group
+
     bf16x8_t wa[KS];
     bf16x8_t wb[NPROD == 3 ? KS : 1];
     {
@@ -160,53 +204,128 @@ __global__ __launch_bounds__(64 * NW) void conv_mfma_kernel(TIO *__restrict__ x,
 #pragma unroll
     for (int r = 0; r < 16; ++r) bias_r[r] = bias[32 * wv + 8 * (r / 4) + 4 * (lane / 32) + (r % 4)];
 
-    for (int j0 = 0; j0 < Lout; j0 += TN) {
-        __syncthreads();  // the previous tile's LDS reads are done
-        // Stage X[ic][j0 .. j0+JW) channel-last, rounded to bf16 (+ residual).
-        for (int e = threadIdx.x; e < C * JW; e += 64 * NW) {
-            const int ic = e / JW, jj = e - ic * JW;
-            const int j = j0 + jj;
-            const float v = j < L ? io_to_f32(xb[(i64)ic * L + j]) : 0.0f;
-            const __bf16 h = to_bf16(v);
-            xs_hi[jj * RS + ic] = h;
-            if constexpr (NPROD == 3) xs_lo[jj * RS + ic] = to_bf16(v - (float)h);
-        }
-        __syncthreads();
-
-        f32x16_t acc[2];
+    TIO pf[TPT][8][4];
+    TIO ph[HPT];
+    auto load_tile = [&](int tl) __attribute__((always_inline)) {
+        const int b = tl / tiles, t = tl - b * tiles;
+        const int j0 = t * TN;
+        const TIO *xb = x + (i64)b * C * L;
 #pragma unroll
-        for (int n = 0; n < 2; ++n)
+        for (int u = 0; u < TPT; ++u) {
+            const int task = tid + u * NT;
+            const int q = task % NQ, g = task / NQ;
+            const int j = j0 + 4 * q;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const TIO *row = xb + (i64)(8 * g + i) * L;
+                if (vec_ok && j + 3 < L) {
+                    const V4 v = *reinterpret_cast<const V4 *>(row + j);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) pf[u][i][c] = __builtin_bit_cast(TIO, v[c]);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) pf[u][i][c] = j + c < L ? row[j + c] : (TIO)0.0f;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < HPT; ++u) {
+            const int e = tid + u * NT;
+            if (e < NH) {
+                const int ic = e / (K - 1), kk = e - ic * (K - 1);
+                if (t < tiles - 1) {
+                    ph[u] = halo[(((i64)b * (tiles - 1) + t) * C + ic) * (K - 1) + kk];
+                } else {
+                    const int j = j0 + TN + kk;
+                    ph[u] = j < L ? xb[(i64)ic * L + j] : (TIO)0.0f;
+                }
+            }
+        }
+    };
+    auto stage_tile = [&](int buf) __attribute__((always_inline)) {
+        __bf16 *hi = xs + (buf * NPART) * BUF;
+        __bf16 *lo = hi + BUF;
+#pragma unroll
+        for (int u = 0; u < TPT; ++u) {
+            const int task = tid + u * NT;
+            const int q = task % NQ, g = task / NQ;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                bf16x8_t hv, lv;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const float f = io_to_f32(pf[u][i][c]);
+                    const __bf16 h = to_bf16(f);
+                    hv[i] = h;
+                    if constexpr (NPROD == 3) lv[i] = to_bf16(f - (float)h);
+                }
+                const int off = (4 * q + c) * RS + 8 * g;
+                *reinterpret_cast<bf16x8_t *>(hi + off) = hv;
+                if constexpr (NPROD == 3) *reinterpret_cast<bf16x8_t *>(lo + off) = lv;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < HPT; ++u) {
+            const int e = tid + u * NT;
+            if (e < NH) {
+                const int ic = e / (K - 1), kk = e - ic * (K - 1);
+                const float f = io_to_f32(ph[u]);
+                const __bf16 h = to_bf16(f);
+                hi[(TN + kk) * RS + ic] = h;
+                if constexpr (NPROD == 3) lo[(TN + kk) * RS + ic] = to_bf16(f - (float)h);
+            }
+        }
+    };
+
+    load_tile(tile);
+    stage_tile(0);
+    __syncthreads();
+    const int jl = lane % 32, kh = 8 * (lane / 32);
+    for (int it = 0; tile < total_tiles; ++it) {
+        const int next = tile + (int)gridDim.x;
+        if (next < total_tiles) load_tile(next);
+        const __bf16 *hi = xs + ((it & 1) * NPART) * BUF;
+        const __bf16 *lo = hi + BUF;
+        f32x16_t acc[NSUB];
+#pragma unroll
+        for (int n = 0; n < NSUB; ++n)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[n][r] = 0.0f;
-        const int jl = lane % 32, kh = 8 * (lane / 32);
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-            const int r0 = 16 * s;          // compile-time after unrolling
+            const int r0 = 16 * s;  // compile-time after unrolling
             const int k = r0 / C, ic0 = r0 - k * C + kh;
 #pragma unroll
-            for (int n = 0; n < 2; ++n) {
+            for (int n = 0; n < NSUB; ++n) {
                 const int row = 32 * n + jl + k;
-                const bf16x8_t bh = *reinterpret_cast<const bf16x8_t *>(xs_hi + row * RS + ic0);
+                const bf16x8_t bh = *reinterpret_cast<const bf16x8_t *>(hi + row * RS + ic0);
                 acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[s], bh, acc[n], 0, 0, 0);
                 if constexpr (NPROD == 3) {
-                    const bf16x8_t bl = *reinterpret_cast<const bf16x8_t *>(xs_lo + row * RS + ic0);
+                    const bf16x8_t bl = *reinterpret_cast<const bf16x8_t *>(lo + row * RS + ic0);
                     acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[s], bl, acc[n], 0, 0, 0);
                     acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wb[s], bh, acc[n], 0, 0, 0);
                 }
             }
         }
         // D[m][n]: lane l, register r -> m = 8*(r/4) + 4*(l/32) + r%4, n = l%32.
+        {
+            const int b = tile / tiles, t = tile - b * tiles;
+            TIO *xb = x + (i64)b * C * L;
 #pragma unroll
-        for (int n = 0; n < 2; ++n) {
-            const int j = j0 + 32 * n + jl;
-            if (j < Lout) {
+            for (int n = 0; n < NSUB; ++n) {
+                const int j = t * TN + 32 * n + jl;
+                if (j < Lout) {
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int oc = 32 * wv + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
-                    xb[(i64)oc * L + j] = f32_to_io<TIO>(acc[n][r] + bias_r[r]);
+                    for (int r = 0; r < 16; ++r) {
+                        const int oc = 32 * wv + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+                        xb[(i64)oc * L + j] = f32_to_io<TIO>(acc[n][r] + bias_r[r]);
+                    }
                 }
             }
         }
+        if (next < total_tiles) stage_tile((it + 1) & 1);
+        __syncthreads();
+        tile = next;
     }
 }
 
@@ -238,12 +357,15 @@ struct ConvPlan {
     void *w = nullptr;      // direct modes: weights in io precision; MFMA: bf16 hi fragments
     void *wlo = nullptr;    // BF16X3: bf16 lo fragments
     void *bias = nullptr;   // direct: io precision; MFMA: f32
+    void *halo = nullptr;   // MFMA: tile-boundary columns, snapshotted before each run
+    i64 resident_groups = 0;
     u64 lds_bytes = 0;
     std::string name;
     ~ConvPlan() {
         if (w) (void)hipFree(w);
         if (wlo) (void)hipFree(wlo);
         if (bias) (void)hipFree(bias);
+        if (halo) (void)hipFree(halo);
     }
 };
 
@@ -254,29 +376,46 @@ template <class Dst, class Src> __global__ void convert_kernel(Dst *dst, const S
 
 using MfmaLaunch = void (*)(void *x, const ConvPlan &p, hipStream_t s);
 
-template <class TIO, int NPROD, int KS, int NW> void launch_mfma(void *x, const ConvPlan &p, hipStream_t s) {
-    auto k = conv_mfma_kernel<TIO, NPROD, KS, NW>;
+template <class TIO, int NPROD, int C, int K, int TN> void launch_mfma(void *x, const ConvPlan &p, hipStream_t s) {
+    const int tiles = (int)ceil_div((u64)(p.L - p.K + 1), (u64)TN);
+    const int total = (int)(p.B * tiles);
+    if (tiles > 1) {
+        const i64 nh = p.B * (tiles - 1) * C * (K - 1);
+        const unsigned g = (unsigned)std::min<i64>(ceil_div((u64)nh, 256ull), 65536);
+        hipLaunchKernelGGL(conv_halo_kernel<TIO>, dim3(g), dim3(256), 0, s, reinterpret_cast<const TIO *>(x),
+                           reinterpret_cast<TIO *>(p.halo), C, (int)p.L, K, TN, tiles, nh);
+    }
+    auto k = conv_mfma_kernel<TIO, NPROD, C, K, TN>;
     (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds_bytes);
-    hipLaunchKernelGGL(k, dim3((unsigned)p.B), dim3(64 * NW), (size_t)p.lds_bytes, s, reinterpret_cast<TIO *>(x),
-                       (const __bf16 *)p.w, (const __bf16 *)p.wlo, (const float *)p.bias, (int)p.L, (int)p.K);
+    const unsigned grid = (unsigned)std::min<i64>(total, p.resident_groups);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(2 * C), (size_t)p.lds_bytes, s, reinterpret_cast<TIO *>(x),
+                       reinterpret_cast<const TIO *>(p.halo), (const __bf16 *)p.w, (const __bf16 *)p.wlo,
+                       (const float *)p.bias, (int)p.L, tiles, total, (int)(p.L % 4 == 0));
+}
+
+template <class TIO, int NPROD, int C, int K, int TN> int occupancy_mfma(u64 lds_bytes) {
+    auto k = conv_mfma_kernel<TIO, NPROD, C, K, TN>;
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 2 * C, (size_t)lds_bytes) != hipSuccess) return 0;
+    return blocks;
 }
 
 struct MfmaVariant {
     bkma_dtype io;
-    int nprod, C, K;
+    int nprod, C, K, TN;
     MfmaLaunch fn;
+    int (*occ)(u64 lds_bytes);
 };
 
+#define MV(TIO, IO, NP, C, K, TN) {IO, NP, C, K, TN, &launch_mfma<TIO, NP, C, K, TN>, &occupancy_mfma<TIO, NP, C, K, TN>}
 const MfmaVariant kMfmaVariants[] = {
-    {BKMA_F32, 1, 128, 5, &launch_mfma<float, 1, 40, 4>},
-    {BKMA_F32, 3, 128, 5, &launch_mfma<float, 3, 40, 4>},
-    {BKMA_BF16, 1, 128, 5, &launch_mfma<__hip_bfloat16, 1, 40, 4>},
-    {BKMA_BF16, 3, 128, 5, &launch_mfma<__hip_bfloat16, 3, 40, 4>},
-    {BKMA_F32, 1, 128, 3, &launch_mfma<float, 1, 24, 4>},
-    {BKMA_F32, 3, 128, 3, &launch_mfma<float, 3, 24, 4>},
-    {BKMA_F32, 1, 64, 5, &launch_mfma<float, 1, 20, 2>},
-    {BKMA_F32, 3, 64, 5, &launch_mfma<float, 3, 20, 2>},
+    MV(float, BKMA_F32, 1, 128, 5, 128),           MV(float, BKMA_F32, 3, 128, 5, 64),
+    MV(__hip_bfloat16, BKMA_BF16, 1, 128, 5, 128), MV(__hip_bfloat16, BKMA_BF16, 3, 128, 5, 64),
+    MV(float, BKMA_F32, 1, 128, 3, 128),           MV(float, BKMA_F32, 3, 128, 3, 64),
+    MV(float, BKMA_F32, 1, 64, 5, 128),            MV(float, BKMA_F32, 3, 64, 5, 64),
 };
+#undef MV
 
 static const MfmaVariant *find_mfma(bkma_dtype io, int nprod, i64 C, i64 K) {
     for (const auto &v : kMfmaVariants)
@@ -330,7 +469,8 @@ extern "C" bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype i
         p->name = want == BKMA_F64 ? "conv_direct<f64>" : "conv_direct<f32>";
     } else if (mode == BKMA_CONV_BF16 || mode == BKMA_CONV_BF16X3) {
         const int nprod = mode == BKMA_CONV_BF16 ? 1 : 3;
-        if (!find_mfma(io_dtype, nprod, (i64)C, (i64)K))
+        const MfmaVariant *mv = find_mfma(io_dtype, nprod, (i64)C, (i64)K);
+        if (!mv)
             return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: no MFMA kernel compiled for C=" + std::to_string(C) +
                                                        ", K=" + std::to_string(K) + " and this io dtype"));
         if (w_dtype != BKMA_F32) return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: MFMA modes take F32 weights"));
@@ -342,9 +482,21 @@ extern "C" bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype i
                            (__bf16 *)p->w, (__bf16 *)p->wlo);
         if (hipMemcpyAsync(p->bias, bias, C * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
             return bail(fail(BKMA_ERR_DEVICE, "conv1d plan: bias copy"));
-        p->lds_bytes = (u64)(64 + K - 1) * (u64)(C + 8) * 2 * (nprod == 3 ? 2 : 1);
+        p->lds_bytes = 2ull * (u64)(mv->TN + K - 1) * (u64)(C + 8) * 2 * (nprod == 3 ? 2 : 1);
+        if (p->lds_bytes > lds_max) return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: LDS tile too large"));
+        const u64 tiles = ceil_div(L - K + 1, (u64)mv->TN);
+        const u64 esz = io_dtype == BKMA_BF16 ? 2 : 4;
+        const u64 hbytes = std::max<u64>(B * (tiles - 1) * C * (K - 1) * esz, 16);
+        if (hipMalloc(&p->halo, hbytes) != hipSuccess)
+            return bail(fail(BKMA_ERR_OUT_OF_MEMORY, "conv1d plan: halo buffer"));
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return bail(fail(BKMA_ERR_DEVICE, "conv1d plan: device query"));
+        p->resident_groups = (i64)std::max(1, mv->occ(p->lds_bytes)) * cus;
         p->name = std::string("conv_mfma_bf16") + (nprod == 3 ? "x3" : "") + "<C" + std::to_string(C) + ",K" +
-                  std::to_string(K) + (io_dtype == BKMA_BF16 ? ",io_bf16" : ",io_f32") + ">";
+                  std::to_string(K) + ",TN" + std::to_string(mv->TN) + (io_dtype == BKMA_BF16 ? ",io_bf16" : ",io_f32") +
+                  ">";
     } else {
         return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: unknown mode"));
     }
