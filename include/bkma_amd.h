@@ -265,12 +265,18 @@ void bkma_adv_batch_destroy(bkma_adv_batch *batch);
  *  BKMA_CONV_F32   : io F32, weights F32; FP32 fused multiply-add
  *  BKMA_CONV_BF16  : io F32 or BF16, weights F32; bf16 MFMA, FP32 accumulate
  *  BKMA_CONV_BF16X3: io F32 or BF16, weights F32; split bf16 (hi*hi + hi*lo
- *                    + lo*hi) MFMA, FP32 accumulate */
+ *                    + lo*hi) MFMA, FP32 accumulate
+ *  BKMA_CONV_FP8   : io F32 or BF16, weights F32; OCP FP8 e4m3 MFMA, FP32
+ *                    accumulate.  Weights are quantized per output channel
+ *                    (scale = the power of two >= max|w[oc]| / 448); the
+ *                    input is quantized as x / input_scale (per tensor, set
+ *                    with bkma_conv1d_plan_set_input_scale, default 1). */
 typedef enum bkma_conv_mode {
     BKMA_CONV_F64 = 0,
     BKMA_CONV_F32 = 1,
     BKMA_CONV_BF16 = 2,
-    BKMA_CONV_BF16X3 = 3
+    BKMA_CONV_BF16X3 = 3,
+    BKMA_CONV_FP8 = 4
 } bkma_conv_mode;
 
 typedef struct bkma_conv1d_plan bkma_conv1d_plan;
@@ -280,10 +286,17 @@ bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype io_dtype, ui
                                     void *stream, bkma_conv1d_plan **out);
 bkma_status bkma_conv1d_plan_run(bkma_conv1d_plan *plan, void *x, void *stream);
 bkma_status bkma_conv1d_plan_describe(const bkma_conv1d_plan *plan, char *name, uint64_t name_len);
+/* FP8 mode: the input is quantized as x / scale (scale > 0; default 1). */
+bkma_status bkma_conv1d_plan_set_input_scale(bkma_conv1d_plan *plan, float scale);
 void bkma_conv1d_plan_destroy(bkma_conv1d_plan *plan);
 /* One v_mfma_f32_32x32x16_bf16 on lane fragments a[64][8], b[64][8] (bf16)
  * into d[64][16] (f32): the check of the operand layout the conv kernels use. */
 bkma_status bkma_mfma_layout_probe(const void *a, const void *b, void *d, void *stream);
+/* One v_mfma_scale_f32_32x32x64_f8f6f4 (FP8 e4m3 operands, unit scales) on
+ * lane fragments a[64][32], b[64][32] (bytes) into d[64][16] (f32). */
+bkma_status bkma_mfma_fp8_layout_probe(const void *a, const void *b, void *d, void *stream);
+/* in[n] (f32, n even) -> out[n] FP8 e4m3 bytes with the FP8 kernels' conversion. */
+bkma_status bkma_fp8_convert_probe(const void *in, void *out, uint64_t n, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Harness kernels: fill_buffer_* (src/tools/init.hpp:31-72) and the   */

@@ -27,7 +27,7 @@ BKMA_ERR_OUT_OF_MEMORY = 5
 BASIC_RANGE, ND_RANGE, ADAPTIVE_WG = 0, 1, 2
 MEM_LOCAL, MEM_GLOBAL = 0, 1
 F64, F32, BF16 = 0, 1, 2
-CONV_F64, CONV_F32, CONV_BF16, CONV_BF16X3 = 0, 1, 2, 3
+CONV_F64, CONV_F32, CONV_BF16, CONV_BF16X3, CONV_FP8 = 0, 1, 2, 3, 4
 
 
 class BkmaError(RuntimeError):
@@ -130,7 +130,9 @@ EXPORTS = [
     "bkma_adv_plan_destroy", "bkma_adv_plan_variants", "bkma_adv_batch_create", "bkma_adv_batch_run",
     "bkma_adv_batch_run_loop", "bkma_adv_batch_describe", "bkma_adv_batch_destroy",
     "bkma_conv1d_plan_create", "bkma_conv1d_plan_run", "bkma_conv1d_plan_describe", "bkma_conv1d_plan_destroy",
-    "bkma_mfma_layout_probe",
+    "bkma_conv1d_plan_set_input_scale",
+    "bkma_conv1d_plan_set_input_scale",
+    "bkma_mfma_layout_probe", "bkma_mfma_fp8_layout_probe", "bkma_fp8_convert_probe",
 ]
 
 
@@ -184,7 +186,10 @@ def lib():
         "bkma_conv1d_plan_run": (st, [vp, vp, vp]),
         "bkma_conv1d_plan_describe": (st, [vp, ctypes.c_char_p, u64]),
         "bkma_conv1d_plan_destroy": (None, [vp]),
+        "bkma_conv1d_plan_set_input_scale": (st, [vp, ctypes.c_float]),
         "bkma_mfma_layout_probe": (st, [vp, vp, vp, vp]),
+        "bkma_mfma_fp8_layout_probe": (st, [vp, vp, vp, vp]),
+        "bkma_fp8_convert_probe": (st, [vp, vp, u64, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -423,6 +428,9 @@ class Conv1dPlan:
     def run(self, x, stream=None) -> None:
         check(lib().bkma_conv1d_plan_run(self._h, _ptr(x), _stream(stream)))
 
+    def set_input_scale(self, scale: float) -> None:
+        check(lib().bkma_conv1d_plan_set_input_scale(self._h, scale))
+
     def name(self) -> str:
         buf = ctypes.create_string_buffer(128)
         check(lib().bkma_conv1d_plan_describe(self._h, buf, 128))
@@ -442,3 +450,11 @@ class Conv1dPlan:
 
 def mfma_layout_probe(a, b, d, stream=None) -> None:
     check(lib().bkma_mfma_layout_probe(_ptr(a), _ptr(b), _ptr(d), _stream(stream)))
+
+
+def mfma_fp8_layout_probe(a, b, d, stream=None) -> None:
+    check(lib().bkma_mfma_fp8_layout_probe(_ptr(a), _ptr(b), _ptr(d), _stream(stream)))
+
+
+def fp8_convert_probe(x, out, stream=None) -> None:
+    check(lib().bkma_fp8_convert_probe(_ptr(x), _ptr(out), x.numel(), _stream(stream)))

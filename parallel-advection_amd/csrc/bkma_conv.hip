@@ -347,6 +347,257 @@ __global__ void mfma_layout_probe_kernel(const __bf16 *a, const __bf16 *b, float
     for (int r = 0; r < 16; ++r) d[l * 16 + r] = c[r];
 }
 
+typedef int i32x8_t __attribute__((ext_vector_type(8)));
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+typedef int i32x2_t __attribute__((ext_vector_type(2)));
+
+// ---------------------------------------------------------------------------
+// FP8 (OCP e4m3) kernels
+// ---------------------------------------------------------------------------
+// Two FP32 values -> two e4m3 bytes (round to nearest even) in the low half.
+__device__ __forceinline__ int pack_fp8x2(float a, float b) { return __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false); }
+
+// Per-output-channel weight scale: the power of two 2^e >= amax/448 (so that
+// |w/scale| <= 448 and integers with few significant bits stay exact).
+__global__ void conv_fp8_weight_scale_kernel(const float *__restrict__ w, int C, int K, float *__restrict__ scale) {
+    const int oc = blockIdx.x;
+    __shared__ float red[256];
+    float m = 0.0f;
+    for (int i = threadIdx.x; i < C * K; i += blockDim.x) m = fmaxf(m, fabsf(w[(i64)oc * C * K + i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float amax = red[0];
+        scale[oc] = amax > 0.0f ? exp2f(ceilf(log2f(amax / 448.0f))) : 1.0f;
+    }
+}
+
+// FP8 W fragments: wave w (channels 32w..), k-step s (r in [64s, 64s+64)),
+// lane l: 32 bytes A[oc = 32w + l%32][r = 64s + 32*(l/32) + e], r = k*C + ic,
+// values w / scale[oc].
+__global__ void conv_fp8_pack_weights_kernel(const float *__restrict__ w, const float *__restrict__ scale, int C,
+                                             int K, unsigned char *__restrict__ out) {
+    const int KS = C * K / 64;
+    const i64 total = (i64)(C / 32) * KS * 64 * 16;  // byte pairs
+    for (i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (i64)gridDim.x * blockDim.x) {
+        const int e2 = (int)(idx % 16);
+        const int l = (int)((idx / 16) % 64);
+        const int s = (int)((idx / 1024) % KS);
+        const int wv = (int)(idx / (1024 * (i64)KS));
+        const int oc = 32 * wv + l % 32;
+        float v[2];
+        for (int h = 0; h < 2; ++h) {
+            const int r = 64 * s + 32 * (l / 32) + 2 * e2 + h;
+            const int k = r / C, ic = r - k * C;
+            v[h] = w[((i64)oc * C + ic) * K + k] / scale[oc];
+        }
+        const int p = pack_fp8x2(v[0], v[1]);
+        out[2 * idx] = (unsigned char)(p & 0xff);
+        out[2 * idx + 1] = (unsigned char)((p >> 8) & 0xff);
+    }
+}
+
+// Persistent FP8 implicit GEMM.  A work-group (C/32 waves) owns all C output
+// channels of a tile of TN columns; wave w computes channels 32w..32w+31 and
+// streams its W fragments (FP8, 32 bytes per lane per k-step of 64) from
+// global memory (L2-resident), one k-step ahead.  The input tile is staged
+// in LDS as FP8 e4m3 (x / xscale), channel-last, double-buffered; the next
+// tile's global loads, conversion and LDS stores are spread over the k-loop
+// of the current one.  y = acc * wscale[oc] * xscale + bias[oc].
+template <class TIO, int C, int K, int TN>
+__global__ __launch_bounds__(2 * C, 1) void conv_fp8_kernel(TIO *__restrict__ x, const TIO *__restrict__ halo,
+                                                          const i32x8_t *__restrict__ wq,
+                                                          const float *__restrict__ wscale,
+                                                          const float *__restrict__ bias, float xscale, int L,
+                                                          int tiles, int total_tiles, int vec_ok) {
+    constexpr int NW = C / 32;
+    constexpr int NT = 64 * NW;
+    constexpr int KS = C * K / 64;
+    constexpr int JW = TN + K - 1;
+    constexpr int RS = C + 16;  // bytes per LDS row
+    constexpr int BUF = JW * RS;
+    constexpr int NSUB = TN / 32;
+    constexpr int NQ = TN / 4;
+    constexpr int NTASK = (C / 8) * NQ;
+    static_assert(NTASK % NT == 0, "staging tasks must divide evenly");
+    constexpr int TPT = NTASK / NT;
+    static_assert(KS % TPT == 0, "k-steps must split evenly over the staging tasks");
+    constexpr int PH = KS / TPT;  // k-steps per staging task
+    constexpr int NH = C * (K - 1);
+    constexpr int HPT = (NH + NT - 1) / NT;
+    using V4 = typename IoVec<TIO, 4>::type;
+
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int Lout = L - K + 1;
+    const float inv_xs = 1.0f / xscale;
+
+    int tile = blockIdx.x;
+    if (tile >= total_tiles) return;  // uniform over the work-group
+
+    float post[16], bias_r[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int oc = 32 * wv + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+        post[r] = wscale[oc] * xscale;
+        bias_r[r] = bias[oc];
+    }
+    const i32x8_t *wsrc = wq + ((i64)wv * KS) * 64 + lane;
+
+    TIO pf[8][4];
+    TIO ph[HPT];
+    auto load_task = [&](int tl, int u) __attribute__((always_inline)) {
+        const int b = tl / tiles, t = tl - b * tiles;
+        const int task = tid + u * NT;
+        const int q = task % NQ, g = task / NQ;
+        const int j = t * TN + 4 * q;
+        const TIO *xb = x + (i64)b * C * L;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const TIO *row = xb + (i64)(8 * g + i) * L;
+            if (vec_ok && j + 3 < L) {
+                const V4 v = *reinterpret_cast<const V4 *>(row + j);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) pf[i][c] = __builtin_bit_cast(TIO, v[c]);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) pf[i][c] = j + c < L ? row[j + c] : (TIO)0.0f;
+            }
+        }
+    };
+    auto store_task = [&](int buf, int u) __attribute__((always_inline)) {
+        unsigned char *xs = lds_raw + buf * BUF;
+        const int task = tid + u * NT;
+        const int q = task % NQ, g = task / NQ;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            i32x2_t v;
+            v[0] = (pack_fp8x2(io_to_f32(pf[0][c]) * inv_xs, io_to_f32(pf[1][c]) * inv_xs) & 0xffff) |
+                   (pack_fp8x2(io_to_f32(pf[2][c]) * inv_xs, io_to_f32(pf[3][c]) * inv_xs) << 16);
+            v[1] = (pack_fp8x2(io_to_f32(pf[4][c]) * inv_xs, io_to_f32(pf[5][c]) * inv_xs) & 0xffff) |
+                   (pack_fp8x2(io_to_f32(pf[6][c]) * inv_xs, io_to_f32(pf[7][c]) * inv_xs) << 16);
+            *reinterpret_cast<i32x2_t *>(xs + (4 * q + c) * RS + 8 * g) = v;
+        }
+    };
+    auto load_halo = [&](int tl) __attribute__((always_inline)) {
+        const int b = tl / tiles, t = tl - b * tiles;
+        const TIO *xb = x + (i64)b * C * L;
+#pragma unroll
+        for (int u = 0; u < HPT; ++u) {
+            const int e = tid + u * NT;
+            if (e < NH) {
+                const int ic = e / (K - 1), kk = e - ic * (K - 1);
+                if (t < tiles - 1) {
+                    ph[u] = halo[(((i64)b * (tiles - 1) + t) * C + ic) * (K - 1) + kk];
+                } else {
+                    const int j = t * TN + TN + kk;
+                    ph[u] = j < L ? xb[(i64)ic * L + j] : (TIO)0.0f;
+                }
+            }
+        }
+    };
+    auto store_halo = [&](int buf) __attribute__((always_inline)) {
+        unsigned char *xs = lds_raw + buf * BUF;
+#pragma unroll
+        for (int u = 0; u < HPT; ++u) {
+            const int e = tid + u * NT;
+            if (e < NH) {
+                const int ic = e / (K - 1), kk = e - ic * (K - 1);
+                xs[(TN + kk) * RS + ic] = (unsigned char)(pack_fp8x2(io_to_f32(ph[u]) * inv_xs, 0.0f) & 0xff);
+            }
+        }
+    };
+
+#pragma unroll
+    for (int u = 0; u < TPT; ++u) {
+        load_task(tile, u);
+        store_task(0, u);
+    }
+    load_halo(tile);
+    store_halo(0);
+    __syncthreads();
+    const int jl = lane % 32, kh = 32 * (lane / 32);
+    for (int it = 0; tile < total_tiles; ++it) {
+        const int next = tile + (int)gridDim.x;
+        const bool has_next = next < total_tiles;
+        const unsigned char *xs = lds_raw + (it & 1) * BUF;
+        f32x16_t acc[NSUB];
+#pragma unroll
+        for (int n = 0; n < NSUB; ++n)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[n][r] = 0.0f;
+        if (has_next) load_halo(next);
+        i32x8_t wcur = wsrc[0];
+        for (int u = 0; u < TPT; ++u) {
+            if (has_next) load_task(next, u);
+#pragma unroll 2
+            for (int ss = 0; ss < PH; ++ss) {
+                const int s = u * PH + ss;
+                const i32x8_t wnext = wsrc[((s + 1) % KS) * 64];  // next k-step (wraps to the next tile's first)
+                const int r0 = 64 * s;
+                const int k = r0 / C, ic0 = r0 - k * C + kh;
+                const unsigned char *src0 = xs + (jl + k) * RS + ic0;
+#pragma unroll
+                for (int n = 0; n < NSUB; ++n) {
+                    const unsigned char *src = src0 + 32 * n * RS;
+                    const i32x4_t b0 = *reinterpret_cast<const i32x4_t *>(src);
+                    const i32x4_t b1 = *reinterpret_cast<const i32x4_t *>(src + 16);
+                    const i32x8_t bv = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+                    acc[n] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(wcur, bv, acc[n], 0, 0, 0, 127, 0, 127);
+                }
+                wcur = wnext;
+            }
+            if (has_next) store_task((it + 1) & 1, u);
+        }
+        if (has_next) store_halo((it + 1) & 1);
+        {
+            const int b = tile / tiles, t = tile - b * tiles;
+            TIO *xb = x + (i64)b * C * L;
+#pragma unroll
+            for (int n = 0; n < NSUB; ++n) {
+                const int j = t * TN + 32 * n + jl;
+                if (j < Lout) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int oc = 32 * wv + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+                        xb[(i64)oc * L + j] = f32_to_io<TIO>(acc[n][r] * post[r] + bias_r[r]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        tile = next;
+    }
+}
+
+
+// One FP8 (OCP e4m3) 32x32x64 MFMA with unit block scales (E8M0 127 = 2^0):
+// a[l][32], b[l][32] bytes are lane l's fragments, d[l][16] the result.
+__global__ void mfma_fp8_layout_probe_kernel(const i32x8_t *a, const i32x8_t *b, float *d) {
+    const int l = threadIdx.x;
+    f32x16_t c;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) c[r] = 0.0f;
+    c = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[l], b[l], c, 0, 0, 0, 127, 0, 127);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) d[l * 16 + r] = c[r];
+}
+
+// FP32 -> FP8 e4m3 with the conversion the FP8 kernels use (pairs, round to
+// nearest even, saturating).
+__global__ void fp8_convert_probe_kernel(const float *in, unsigned char *out, i64 npairs) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < npairs; i += (i64)gridDim.x * blockDim.x) {
+        const int packed = __builtin_amdgcn_cvt_pk_fp8_f32(in[2 * i], in[2 * i + 1], 0, false);
+        out[2 * i] = (unsigned char)(packed & 0xff);
+        out[2 * i + 1] = (unsigned char)((packed >> 8) & 0xff);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -358,6 +609,8 @@ struct ConvPlan {
     void *wlo = nullptr;    // BF16X3: bf16 lo fragments
     void *bias = nullptr;   // direct: io precision; MFMA: f32
     void *halo = nullptr;   // MFMA: tile-boundary columns, snapshotted before each run
+    void *wscale = nullptr; // FP8: per-output-channel weight scales (f32)
+    float xscale = 1.0f;    // FP8: the input is quantized as x / xscale
     i64 resident_groups = 0;
     u64 lds_bytes = 0;
     std::string name;
@@ -366,6 +619,7 @@ struct ConvPlan {
         if (wlo) (void)hipFree(wlo);
         if (bias) (void)hipFree(bias);
         if (halo) (void)hipFree(halo);
+        if (wscale) (void)hipFree(wscale);
     }
 };
 
@@ -376,15 +630,12 @@ template <class Dst, class Src> __global__ void convert_kernel(Dst *dst, const S
 
 using MfmaLaunch = void (*)(void *x, const ConvPlan &p, hipStream_t s);
 
+template <class TIO> void launch_halo(void *x, const ConvPlan &p, int C, int K, int TN, int tiles, hipStream_t s);
+
 template <class TIO, int NPROD, int C, int K, int TN> void launch_mfma(void *x, const ConvPlan &p, hipStream_t s) {
     const int tiles = (int)ceil_div((u64)(p.L - p.K + 1), (u64)TN);
     const int total = (int)(p.B * tiles);
-    if (tiles > 1) {
-        const i64 nh = p.B * (tiles - 1) * C * (K - 1);
-        const unsigned g = (unsigned)std::min<i64>(ceil_div((u64)nh, 256ull), 65536);
-        hipLaunchKernelGGL(conv_halo_kernel<TIO>, dim3(g), dim3(256), 0, s, reinterpret_cast<const TIO *>(x),
-                           reinterpret_cast<TIO *>(p.halo), C, (int)p.L, K, TN, tiles, nh);
-    }
+    launch_halo<TIO>(x, p, C, K, TN, tiles, s);
     auto k = conv_mfma_kernel<TIO, NPROD, C, K, TN>;
     (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds_bytes);
     const unsigned grid = (unsigned)std::min<i64>(total, p.resident_groups);
@@ -416,6 +667,48 @@ const MfmaVariant kMfmaVariants[] = {
     MV(float, BKMA_F32, 1, 64, 5, 128),            MV(float, BKMA_F32, 3, 64, 5, 64),
 };
 #undef MV
+
+template <class TIO> void launch_halo(void *x, const ConvPlan &p, int C, int K, int TN, int tiles, hipStream_t s) {
+    if (tiles <= 1) return;
+    const i64 nh = p.B * (tiles - 1) * C * (K - 1);
+    const unsigned g = (unsigned)std::min<i64>(ceil_div((u64)nh, 256ull), 65536);
+    hipLaunchKernelGGL(conv_halo_kernel<TIO>, dim3(g), dim3(256), 0, s, reinterpret_cast<const TIO *>(x),
+                       reinterpret_cast<TIO *>(p.halo), C, (int)p.L, K, TN, tiles, nh);
+}
+
+template <class TIO, int C, int K, int TN> void launch_fp8(void *x, const ConvPlan &p, hipStream_t s) {
+    const int tiles = (int)ceil_div((u64)(p.L - p.K + 1), (u64)TN);
+    const int total = (int)(p.B * tiles);
+    launch_halo<TIO>(x, p, C, K, TN, tiles, s);
+    auto k = conv_fp8_kernel<TIO, C, K, TN>;
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds_bytes);
+    const unsigned grid = (unsigned)std::min<i64>(total, p.resident_groups);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(2 * C), (size_t)p.lds_bytes, s, reinterpret_cast<TIO *>(x),
+                       reinterpret_cast<const TIO *>(p.halo), (const i32x8_t *)p.w, (const float *)p.wscale,
+                       (const float *)p.bias, p.xscale, (int)p.L, tiles, total, (int)(p.L % 4 == 0));
+}
+
+template <class TIO, int C, int K, int TN> int occupancy_fp8(u64 lds_bytes) {
+    auto k = conv_fp8_kernel<TIO, C, K, TN>;
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 2 * C, (size_t)lds_bytes) != hipSuccess) return 0;
+    return blocks;
+}
+
+#define FV(TIO, IO, C, K, TN) {IO, 0, C, K, TN, &launch_fp8<TIO, C, K, TN>, &occupancy_fp8<TIO, C, K, TN>}
+const MfmaVariant kFp8Variants[] = {
+    FV(float, BKMA_F32, 256, 7, 128),
+    FV(__hip_bfloat16, BKMA_BF16, 256, 7, 128),
+    FV(float, BKMA_F32, 128, 5, 128),
+};
+#undef FV
+
+static const MfmaVariant *find_fp8(bkma_dtype io, i64 C, i64 K) {
+    for (const auto &v : kFp8Variants)
+        if (v.io == io && v.C == C && v.K == K) return &v;
+    return nullptr;
+}
 
 static const MfmaVariant *find_mfma(bkma_dtype io, int nprod, i64 C, i64 K) {
     for (const auto &v : kMfmaVariants)
@@ -497,6 +790,34 @@ extern "C" bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype i
         p->name = std::string("conv_mfma_bf16") + (nprod == 3 ? "x3" : "") + "<C" + std::to_string(C) + ",K" +
                   std::to_string(K) + ",TN" + std::to_string(mv->TN) + (io_dtype == BKMA_BF16 ? ",io_bf16" : ",io_f32") +
                   ">";
+    } else if (mode == BKMA_CONV_FP8) {
+        const MfmaVariant *mv = find_fp8(io_dtype, (i64)C, (i64)K);
+        if (!mv)
+            return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: no FP8 kernel compiled for C=" + std::to_string(C) +
+                                                       ", K=" + std::to_string(K) + " and this io dtype"));
+        if (w_dtype != BKMA_F32) return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: FP8 mode takes F32 weights"));
+        if (hipMalloc(&p->w, (size_t)nw) != hipSuccess || hipMalloc(&p->wscale, C * 4) != hipSuccess ||
+            hipMalloc(&p->bias, C * 4) != hipSuccess)
+            return bail(fail(BKMA_ERR_OUT_OF_MEMORY, "conv1d plan: weight fragments"));
+        hipLaunchKernelGGL(conv_fp8_weight_scale_kernel, dim3((unsigned)C), dim3(256), 0, s, (const float *)weight,
+                           (int)C, (int)K, (float *)p->wscale);
+        hipLaunchKernelGGL(conv_fp8_pack_weights_kernel, dim3(256), dim3(256), 0, s, (const float *)weight,
+                           (const float *)p->wscale, (int)C, (int)K, (unsigned char *)p->w);
+        if (hipMemcpyAsync(p->bias, bias, C * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            return bail(fail(BKMA_ERR_DEVICE, "conv1d plan: bias copy"));
+        p->lds_bytes = 2ull * (u64)(mv->TN + K - 1) * (u64)(C + 16);
+        if (p->lds_bytes > lds_max) return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: LDS tile too large"));
+        const u64 tiles = ceil_div(L - K + 1, (u64)mv->TN);
+        const u64 esz = io_dtype == BKMA_BF16 ? 2 : 4;
+        if (hipMalloc(&p->halo, std::max<u64>(B * (tiles - 1) * C * (K - 1) * esz, 16)) != hipSuccess)
+            return bail(fail(BKMA_ERR_OUT_OF_MEMORY, "conv1d plan: halo buffer"));
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return bail(fail(BKMA_ERR_DEVICE, "conv1d plan: device query"));
+        p->resident_groups = (i64)std::max(1, mv->occ(p->lds_bytes)) * cus;
+        p->name = std::string("conv_fp8_e4m3<C") + std::to_string(C) + ",K" + std::to_string(K) + ",TN" +
+                  std::to_string(mv->TN) + (io_dtype == BKMA_BF16 ? ",io_bf16" : ",io_f32") + ">";
     } else {
         return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: unknown mode"));
     }
@@ -530,6 +851,12 @@ extern "C" bkma_status bkma_conv1d_plan_run(bkma_conv1d_plan *p, void *x, void *
             v->fn(x, *p, s);
             break;
         }
+        case BKMA_CONV_FP8: {
+            const MfmaVariant *v = find_fp8(p->io, p->C, p->K);
+            if (!v) return fail(BKMA_ERR_UNSUPPORTED, "conv1d run: no FP8 kernel");
+            v->fn(x, *p, s);
+            break;
+        }
         default:
             return fail(BKMA_ERR_UNSUPPORTED, "conv1d run: unknown mode");
     }
@@ -545,7 +872,32 @@ extern "C" bkma_status bkma_conv1d_plan_describe(const bkma_conv1d_plan *p, char
     return BKMA_OK;
 }
 
+extern "C" bkma_status bkma_conv1d_plan_set_input_scale(bkma_conv1d_plan *p, float scale) {
+    if (!p) return fail(BKMA_ERR_INVALID_ARGUMENT, "null plan");
+    if (!(scale > 0.0f) || !std::isfinite(scale)) return fail(BKMA_ERR_INVALID_ARGUMENT, "input scale must be positive");
+    p->xscale = scale;
+    return BKMA_OK;
+}
+
 extern "C" void bkma_conv1d_plan_destroy(bkma_conv1d_plan *p) { delete p; }
+
+extern "C" bkma_status bkma_mfma_fp8_layout_probe(const void *a, const void *b, void *d, void *stream) {
+    if (!a || !b || !d) return fail(BKMA_ERR_INVALID_ARGUMENT, "null argument");
+    BKMA_TRY(require_device());
+    hipLaunchKernelGGL(mfma_fp8_layout_probe_kernel, dim3(1), dim3(64), 0, as_stream(stream), (const i32x8_t *)a,
+                       (const i32x8_t *)b, (float *)d);
+    return check_launch("mfma_fp8_layout_probe_kernel");
+}
+
+extern "C" bkma_status bkma_fp8_convert_probe(const void *in, void *out, uint64_t n, void *stream) {
+    if (!in || !out || n % 2) return fail(BKMA_ERR_INVALID_ARGUMENT, "fp8 convert probe: null argument or odd n");
+    BKMA_TRY(require_device());
+    const i64 npairs = (i64)(n / 2);
+    const unsigned g = (unsigned)std::min<i64>(std::max<i64>(1, (npairs + 255) / 256), 4096);
+    hipLaunchKernelGGL(fp8_convert_probe_kernel, dim3(g), dim3(256), 0, as_stream(stream), (const float *)in,
+                       (unsigned char *)out, npairs);
+    return check_launch("fp8_convert_probe_kernel");
+}
 
 extern "C" bkma_status bkma_mfma_layout_probe(const void *a, const void *b, void *d, void *stream) {
     if (!a || !b || !d) return fail(BKMA_ERR_INVALID_ARGUMENT, "null argument");

@@ -108,3 +108,118 @@ def test_conv_unsupported_shape_is_reported(cuda):
     b = torch.zeros(96, dtype=torch.float32, device="cuda")
     with pytest.raises(B.BkmaUnsupported):
         B.Conv1dPlan(B.CONV_BF16, B.F32, 2, 96, 100, 5, w, b)
+
+
+# ---------------------------------------------------------------------------
+# FP8 (OCP e4m3) building blocks of the FP8 kernels
+# ---------------------------------------------------------------------------
+def test_fp8_conversion_matches_ocp_e4m3(cuda):
+    import fp8_e4m3 as F8
+    torch = _torch()
+    rng = np.random.default_rng(5)
+    grid = F8.DECODE[np.isfinite(F8.DECODE)]
+    mids = (np.sort(grid)[1:] + np.sort(grid)[:-1]) / 2  # exact ties between neighbours
+    x = np.concatenate([rng.uniform(-448, 448, 4000), rng.standard_normal(4000) * 1e-2, grid, mids,
+                        rng.standard_normal(4000)]).astype(np.float32)
+    x = x[: len(x) // 2 * 2]
+    tx = torch.from_numpy(x).cuda()
+    out = torch.empty(len(x), dtype=torch.uint8, device="cuda")
+    B.fp8_convert_probe(tx, out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    exp = F8.encode(x.astype(np.float64))
+    bad = np.nonzero(got != exp)[0]
+    assert bad.size == 0, [(float(x[i]), int(got[i]), int(exp[i])) for i in bad[:8]]
+
+
+def test_mfma_32x32x64_fp8_layout(cuda):
+    # Lane l holds A[m = l%32][k = 32*(l/32) + e] and B[k = 32*(l/32) + e][n = l%32],
+    # e = 0..31; D as for the bf16 32x32 MFMA.  Small integers are exact in e4m3
+    # and their 64-term sums are exact in FP32.
+    import fp8_e4m3 as F8
+    torch = _torch()
+    rng = np.random.default_rng(6)
+    A = rng.integers(-4, 5, (32, 64)).astype(np.float64)
+    Bm = rng.integers(-4, 5, (64, 32)).astype(np.float64)
+    a = np.empty((64, 32), np.uint8)
+    b = np.empty((64, 32), np.uint8)
+    for l in range(64):
+        for e in range(32):
+            a[l, e] = F8.encode(A[l % 32, 32 * (l // 32) + e])
+            b[l, e] = F8.encode(Bm[32 * (l // 32) + e, l % 32])
+    ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    d = torch.empty((64, 16), dtype=torch.float32, device="cuda")
+    B.mfma_fp8_layout_probe(ta, tb, d)
+    torch.cuda.synchronize()
+    dh = d.cpu().numpy()
+    got = np.empty((32, 32))
+    for l in range(64):
+        for r in range(16):
+            got[8 * (r // 4) + 4 * (l // 32) + r % 4, l % 32] = dh[l, r]
+    assert np.array_equal(got, A @ Bm)
+
+
+# ---------------------------------------------------------------------------
+# FP8 conv plans
+# ---------------------------------------------------------------------------
+FP8_SHAPES = [(2, 128, 300, 5), (2, 256, 301, 7), (1, 256, 140, 7)]
+
+
+@pytest.mark.parametrize("shape", FP8_SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("xscale", [1.0, 2.0])
+def test_conv_fp8_exact_on_representable_inputs(cuda, shape, xscale):
+    # Small integers: every weight (scaled by its channel's power of two) and
+    # every input (divided by the power-of-two input scale) is exact in e4m3,
+    # and the FP32 sums stay below 2^24, so the result equals the FP64 oracle.
+    torch = _torch()
+    Bn, C, L, K = shape
+    rng = np.random.default_rng(7)
+    x = rng.integers(-3, 4, (Bn, C, L)).astype(np.float64)
+    w = rng.integers(-4, 5, (C, C, K)).astype(np.float64)
+    w[:, 0, 0] = 4.0  # per-channel max |w| = 4 -> scale 2^-6
+    b = rng.integers(-8, 9, C).astype(np.float64)
+    tx = torch.from_numpy(x).float().cuda()
+    plan = B.Conv1dPlan(B.CONV_FP8, B.F32, Bn, C, L, K, torch.from_numpy(w).float().cuda(),
+                        torch.from_numpy(b).float().cuda())
+    plan.set_input_scale(xscale)
+    print(plan.name(), flush=True)
+    plan.run(tx)
+    torch.cuda.synchronize()
+    got = tx.double().cpu().numpy()
+    ref = _ref(x, w, b)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("io", [B.F32, B.BF16], ids=["io32", "io16"])
+def test_conv_fp8_matches_quantized_emulation(cuda, io):
+    # N(0,1) data: the kernel must equal the FP64 convolution of the e4m3-
+    # quantized operands (OCP rounding, per-output-channel power-of-two weight
+    # scales) up to FP32 accumulation; the error against the unquantized FP64
+    # result is printed.
+    import fp8_e4m3 as F8
+    torch = _torch()
+    Bn, C, L, K = 2, 256, 400, 7
+    x, w, b = _case(Bn, C, L, K, 8)
+    w32 = w.astype(np.float32).astype(np.float64)
+    b32 = b.astype(np.float32).astype(np.float64)
+    tdt = torch.float32 if io == B.F32 else torch.bfloat16
+    tx = torch.from_numpy(x).to(tdt).cuda()
+    x_in = tx.double().cpu().numpy()
+    plan = B.Conv1dPlan(B.CONV_FP8, io, Bn, C, L, K, torch.from_numpy(w).float().cuda(),
+                        torch.from_numpy(b).float().cuda())
+    plan.run(tx)
+    torch.cuda.synchronize()
+    got = tx.double().cpu().numpy()
+    amax = np.abs(w32).reshape(C, -1).max(axis=1)
+    scale = 2.0 ** np.ceil(np.log2(amax / 448.0))
+    wq = F8.decode(F8.encode(w32 / scale[:, None, None])) * scale[:, None, None]
+    xq = F8.decode(F8.encode(x_in))
+    emu = _ref(xq, wq, b32)
+    exact = _ref(x_in, w32, b32)
+    Lout = L - K + 1
+    e_max, e_l2 = _rel(got, emu, Lout)
+    q_max, q_l2 = _rel(got, exact, Lout)
+    print(f"vs quantized emulation rel_max {e_max:.3e}; vs FP64 rel_max {q_max:.3e} rel_L2 {q_l2:.3e}", flush=True)
+    assert e_max <= (5e-6 if io == B.F32 else 8e-3)  # bf16 storage rounds the outputs
+    assert np.array_equal(got[:, :, Lout:], x_in[:, :, Lout:])
+    assert q_l2 < 0.1
