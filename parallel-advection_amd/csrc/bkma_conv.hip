@@ -146,6 +146,14 @@ __global__ void conv_halo_kernel(const TIO *__restrict__ x, TIO *__restrict__ ha
     }
 }
 
+// Element c of a vector load as the io type.  The element is copied out
+// first: a bit_cast applied to the subscript expression itself read element
+// 0 for every c with this compiler.
+template <class TIO, class V> __device__ __forceinline__ TIO io_elem(const V &v, int c) {
+    const auto e = v[c];
+    return __builtin_bit_cast(TIO, e);
+}
+
 template <class T, int N> struct IoVec;
 template <> struct IoVec<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
 template <> struct IoVec<__hip_bfloat16, 4> { typedef unsigned short type __attribute__((ext_vector_type(4))); };
@@ -221,7 +229,7 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
                 if (vec_ok && j + 3 < L) {
                     const V4 v = *reinterpret_cast<const V4 *>(row + j);
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) pf[u][i][c] = __builtin_bit_cast(TIO, v[c]);
+                    for (int c = 0; c < 4; ++c) pf[u][i][c] = io_elem<TIO>(v, c);
                 } else {
 #pragma unroll
                     for (int c = 0; c < 4; ++c) pf[u][i][c] = j + c < L ? row[j + c] : (TIO)0.0f;
@@ -463,7 +471,7 @@ __global__ __launch_bounds__(2 * C, 1) void conv_fp8_kernel(TIO *__restrict__ x,
             if (vec_ok && j + 3 < L) {
                 const V4 v = *reinterpret_cast<const V4 *>(row + j);
 #pragma unroll
-                for (int c = 0; c < 4; ++c) pf[i][c] = __builtin_bit_cast(TIO, v[c]);
+                for (int c = 0; c < 4; ++c) pf[i][c] = io_elem<TIO>(v, c);
             } else {
 #pragma unroll
                 for (int c = 0; c < 4; ++c) pf[i][c] = j + c < L ? row[j + c] : (TIO)0.0f;

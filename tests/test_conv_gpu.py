@@ -79,9 +79,10 @@ def _rel(got, ref, Lout):
                                          (B.CONV_BF16X3, B.BF16, 8e-3)],
                          ids=["f32", "bf16_io32", "bf16x3_io32", "bf16_io16", "bf16x3_io16"])
 @pytest.mark.parametrize("K", [5, 3])
-def test_conv_reduced_precision(cuda, mode, io, tol, K):
+@pytest.mark.parametrize("L", [517, 516], ids=["L517", "L516"])  # scalar and vector staging loads
+def test_conv_reduced_precision(cuda, mode, io, tol, K, L):
     torch = _torch()
-    Bn, C, L = 3, 128, 517
+    Bn, C = 3, 128
     if io == B.BF16 and K != 5:
         pytest.skip("bf16 storage kernels are compiled for K = 5")
     x, w, b = _case(Bn, C, L, K, 2)
@@ -162,7 +163,7 @@ def test_mfma_32x32x64_fp8_layout(cuda):
 # ---------------------------------------------------------------------------
 # FP8 conv plans
 # ---------------------------------------------------------------------------
-FP8_SHAPES = [(2, 128, 300, 5), (2, 256, 301, 7), (1, 256, 140, 7)]
+FP8_SHAPES = [(2, 128, 300, 5), (2, 128, 301, 5), (2, 256, 301, 7), (1, 256, 140, 7), (3, 256, 1024, 7)]
 
 
 @pytest.mark.parametrize("shape", FP8_SHAPES, ids=lambda s: "x".join(map(str, s)))
@@ -220,6 +221,48 @@ def test_conv_fp8_matches_quantized_emulation(cuda, io):
     e_max, e_l2 = _rel(got, emu, Lout)
     q_max, q_l2 = _rel(got, exact, Lout)
     print(f"vs quantized emulation rel_max {e_max:.3e}; vs FP64 rel_max {q_max:.3e} rel_L2 {q_l2:.3e}", flush=True)
-    assert e_max <= (5e-6 if io == B.F32 else 8e-3)  # bf16 storage rounds the outputs
+    # FP32 accumulation of C*K = 1792 exact products: the rounding error stays
+    # below ~1e-4 of the largest output; bf16 storage rounds the outputs.
+    assert e_max <= (1e-4 if io == B.F32 else 8e-3)
     assert np.array_equal(got[:, :, Lout:], x_in[:, :, Lout:])
     assert q_l2 < 0.1
+
+
+# More tiles than resident work-groups: the persistent loop, the prefetch of
+# the next tile into the other LDS buffer and the halo snapshots all run.
+@pytest.mark.parametrize("mode,io,tol", [(B.CONV_BF16, B.F32, 1e-2), (B.CONV_BF16X3, B.F32, 1e-4),
+                                         (B.CONV_BF16, B.BF16, 1e-2)], ids=["bf16_io32", "bf16x3_io32", "bf16_io16"])
+def test_conv_mfma_persistent_many_tiles(cuda, mode, io, tol):
+    torch = _torch()
+    Bn, C, L, K = 40, 128, 1024, 5
+    x, w, b = _case(Bn, C, L, K, 9)
+    tdt = torch.float32 if io == B.F32 else torch.bfloat16
+    tx = torch.from_numpy(x).to(tdt).cuda()
+    x_in = tx.double().cpu().numpy()
+    plan = B.Conv1dPlan(mode, io, Bn, C, L, K, torch.from_numpy(w).float().cuda(), torch.from_numpy(b).float().cuda())
+    plan.run(tx)
+    torch.cuda.synchronize()
+    got = tx.double().cpu().numpy()
+    ref = _ref(x_in, w.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64))
+    Lout = L - K + 1
+    rmax, rl2 = _rel(got, ref, Lout)
+    print(plan.name(), f"rel_max {rmax:.3e} rel_L2 {rl2:.3e}", flush=True)
+    assert rmax <= tol
+    assert np.array_equal(got[:, :, Lout:], x_in[:, :, Lout:])
+
+
+@pytest.mark.parametrize("shape", [(40, 128, 1024, 5), (20, 256, 1024, 7)], ids=lambda s: "x".join(map(str, s)))
+def test_conv_fp8_persistent_many_tiles_exact(cuda, shape):
+    torch = _torch()
+    Bn, C, L, K = shape
+    rng = np.random.default_rng(10)
+    x = rng.integers(-3, 4, (Bn, C, L)).astype(np.float64)
+    w = rng.integers(-4, 5, (C, C, K)).astype(np.float64)
+    w[:, 0, 0] = 4.0
+    b = rng.integers(-8, 9, C).astype(np.float64)
+    tx = torch.from_numpy(x).float().cuda()
+    plan = B.Conv1dPlan(B.CONV_FP8, B.F32, Bn, C, L, K, torch.from_numpy(w).float().cuda(),
+                        torch.from_numpy(b).float().cuda())
+    plan.run(tx)
+    torch.cuda.synchronize()
+    assert np.array_equal(tx.double().cpu().numpy(), _ref(x, w, b))
