@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (3 launches), for scaling rocprofv3 byte counters")
     ap.add_argument("--extra", default="",
-                    help="comma list of extra measurements added to the JSON line: conv, batch")
+                    help="comma list of extra measurements added to the JSON line: conv, batch, conv_fp8, long")
     return ap.parse_args()
 
 
@@ -218,6 +218,10 @@ def main():
         out["conv1d"] = measure_conv(torch, B)
     if "batch" in extras:
         out["ragged_batch"] = measure_batch(torch, B)
+    if "conv_fp8" in extras:
+        out["conv1d_fp8"] = measure_conv_fp8(torch, B)
+    if "long" in extras:
+        out["long_lines"] = measure_long(torch, B)
     print(json.dumps(out), flush=True)
 
 
@@ -328,22 +332,132 @@ def measure_batch(torch, B):
     return out
 
 
+def _mix64(z):
+    m = (1 << 64) - 1
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
+
+
+def _uniform(seed, idx):
+    """U[0,1) keyed by (seed, idx): the library's counter-based generator
+    (SplitMix64 finalizer over a Weyl sequence, 53 random bits)."""
+    m = (1 << 64) - 1
+    key = _mix64(seed ^ 0x5851F42D4C957F2D)
+    return (_mix64((key + (idx + 1) * 0x9E3779B97F4A7C15) & m) >> 11) * 2.0 ** -53
+
+
 def ragged_specs(seed, count, n1_lo=64, n1_hi=16384, n0_lo=16, n0_hi=1024):
-    """Shapes of the ragged batch from the counter-based generator: problem j
-    uses draws 4j..4j+3 of `seed`."""
+    """Shapes of the ragged batch: problem j uses draws 4j..4j+3 of `seed`."""
     import math
-    import bkma_amd as B  # noqa: F401  (generator constants live in the oracle and the library)
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
     out = []
     for j in range(count):
-        u = [O.lib().orc_uniform(seed, 4 * j + k) for k in range(4)]
+        u = [_uniform(seed, 4 * j + k) for k in range(4)]
         n1 = int(round(math.exp(math.log(n1_lo) + u[0] * (math.log(n1_hi) - math.log(n1_lo))) / 64)) * 64
         n1 = max(64, min(n1_hi, n1))
         n0 = n0_lo + int(u[1] * (n0_hi - n0_lo + 1))
         n2 = (1, 8, 64)[min(2, int(u[2] * 3))]
         dt = 5e-4 + u[3] * 1.5e-3
         out.append((n0, n1, n2, dt))
+    return out
+
+
+def measure_conv_fp8(torch, B):
+    """Conv1d x[512][256][8192], w[256][256][7], x and w ~ N(0,1): FP8 e4m3
+    MFMA (per-output-channel weight scales, unit input scale, FP32
+    accumulation) with FP32 and BF16 storage; errors against the FP64 direct
+    plan on the same inputs."""
+    Bn, C, L, K = 512, 256, 8192, 7
+    flop = 2.0 * Bn * C * C * K * (L - K + 1)
+    w = torch.empty((C, C, K), dtype=torch.float64, device="cuda")
+    B.fill_normal(w, 41, 1.0)
+    b = torch.zeros(C, dtype=torch.float64, device="cuda")
+    x64 = torch.empty((Bn, C, L), dtype=torch.float64, device="cuda")
+    B.fill_normal(x64, 4)
+    x32 = x64.float()
+    del x64
+    torch.cuda.empty_cache()
+    res = {"workload": f"x[{Bn}][{C}][{L}] w[{C}][{C}][{K}] in place, N(0,1)", "flop_per_call": flop}
+    wf, bf = w.float(), b.float()
+    Lout = L - K + 1
+    for name, io in (("fp8_io_f32", B.F32), ("fp8_io_bf16", B.BF16)):
+        plan = B.Conv1dPlan(B.CONV_FP8, io, Bn, C, L, K, wf, bf)
+        xin = x32 if io == B.F32 else x32.to(torch.bfloat16)
+        x = xin.clone()
+        plan.run(x)
+        torch.cuda.synchronize()
+        ref = xin.double()
+        p64 = B.Conv1dPlan(B.CONV_F64, B.F64, Bn, C, L, K, w.double(), b.double())
+        p64.run(ref)
+        torch.cuda.synchronize()
+        d = x[:, :, :Lout].double() - ref[:, :, :Lout]
+        rel_max = (d.abs().max() / ref[:, :, :Lout].abs().max()).item()
+        rel_l2 = (d.norm() / ref[:, :, :Lout].norm()).item()
+        del d, ref, p64
+        torch.cuda.empty_cache()
+        reps = 5
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            plan.run(x)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res[name] = {"kernel": plan.name(), "ms": round(ms, 4), "TFLOPs": round(flop / ms / 1e9, 2),
+                     "frac_of_dense_fp8_peak": round(flop / ms / 1e9 / 5000.0, 4),
+                     "rel_max": rel_max, "rel_L2": rel_l2}
+        plan.close()
+        del x
+        torch.cuda.empty_cache()
+    return res
+
+
+def measure_long(torch, B):
+    """Long-line advection f[256][262144][4] (lines of 1 MiB FP32 / 2 MiB FP64,
+    beyond LDS), U(0,1) seed 4, 20 steps, v in [-1,1), dt = 1e-3 (shift about
+    262 cells): autotuned FP64 and FP32 plans; the FP32 error against FP64."""
+    n0, n1, n2, steps = 256, 262144, 4, 20
+    cells = n0 * n1 * n2
+    p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    out = {"workload": f"f[{n0}][{n1}][{n2}] along n1, 20 steps", "cells": cells}
+    f64 = torch.empty((n0, n1, n2), dtype=torch.float64, device="cuda")
+    B.fill_uniform(f64, 4)
+    f32 = f64.float()
+    for key, arr, dt in (("fp64", f64, B.F64), ("fp32", f32, B.F32)):
+        plan = B.AdvPlan(p, dt, autotune=True)
+        _, name = plan.describe()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.run(arr, steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        esz = arr.element_size()
+        out[key] = {"kernel": name, "s": round(el, 4), "Gcell_s": round(cells * steps / el / 1e9, 2),
+                    "GBps_alg": round(cells * steps * 2 * esz / el / 1e9, 1)}
+        plan.close()
+    rel_max, rel_l2 = B.rel_errors(f32, f64)
+    out["error_fp32_vs_fp64"] = {"rel_max": rel_max, "rel_L2": rel_l2}
+    try:
+        g = torch.empty_like(f64)
+        B.fill_uniform(g, 4)
+        o = B.create_optim_params(n0, n1, n2, pref_wg_size=256, elem_size=8)
+        scratch, ext = None, None
+        if o.mem_space == B.MEM_GLOBAL:
+            ext = B.scratch_extents(B.ADAPTIVE_WG, o, n0, n1, n2, n1)
+            scratch = torch.empty(ext[0] * ext[1] * ext[2], dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(2):
+            B.adv_run(B.ADAPTIVE_WG, g, p, o, scratch=scratch, scratch_extents=ext)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / 2
+        out["plain_fp64_gpu"] = {"s_per_step": round(el, 5), "Gcell_s": round(cells / el / 1e9, 2),
+                                 "mem_space": "global" if o.mem_space == B.MEM_GLOBAL else "local"}
+        del g, scratch
+    except Exception as e:  # the reference's policy may not cover this shape
+        out["plain_fp64_gpu"] = {"error": str(e)}
+    del f64, f32
+    torch.cuda.empty_cache()
     return out
 
 

@@ -332,13 +332,18 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
     return out;
 }
 
-// Heuristic: column stream (smallest window, most steps per pass, widest
-// vector, deepest prefetch) when it applies, else LINE_LDS whole line, else
-// the longest segment.
-int heuristic_score(const bkma_tuned_config &c) {
-    if (c.strategy == BKMA_STRATEGY_COLUMN_STREAM)
-        return 1000000000 - c.smax * 1000000 + steps_per_pass(c) * 10000 + c.vec * 1000 + c.prefetch * 10 +
+// Heuristic (measured on the strided 2048x1024x512 workload): column stream
+// with the smallest window, FP32 at 2 and FP64 at 3 steps per pass (more
+// steps push the register window past the occupancy that hides HBM latency),
+// widest vector, deepest prefetch; else LINE_LDS whole line, else the
+// longest segment.
+int heuristic_score(const bkma_tuned_config &c, bkma_dtype dt) {
+    if (c.strategy == BKMA_STRATEGY_COLUMN_STREAM) {
+        const int want = dt == BKMA_F64 ? 3 : 2;
+        const int ns = steps_per_pass(c);
+        return 1000000000 - c.smax * 1000000 - std::abs(ns - want) * 100000 + c.vec * 1000 + c.prefetch * 10 +
                c.nontemporal - c.persistent - 100 * c.wide_window;
+    }
     return c.seg_rows == 0 ? 500000000 : 100000000 + std::min(c.seg_rows, 1 << 24) / 1024;
 }
 
@@ -381,8 +386,8 @@ extern "C" bkma_status bkma_adv_plan_create(const bkma_adv_params *params, bkma_
     std::vector<bkma_tuned_config> cands = candidates(*params, dtype, &why);
     if (cands.empty())
         return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: no strategy covers this shape (" + why + ")");
-    bkma_tuned_config best = *std::max_element(cands.begin(), cands.end(), [](const auto &a, const auto &b) {
-        return heuristic_score(a) < heuristic_score(b);
+    bkma_tuned_config best = *std::max_element(cands.begin(), cands.end(), [&](const auto &a, const auto &b) {
+        return heuristic_score(a, dtype) < heuristic_score(b, dtype);
     });
 
     if (autotune && cands.size() > 1) {
