@@ -443,7 +443,7 @@ def measure_long(torch, B):
         o = B.create_optim_params(n0, n1, n2, pref_wg_size=256, elem_size=8)
         scratch, ext = None, None
         if o.mem_space == B.MEM_GLOBAL:
-            ext = B.scratch_extents(B.ADAPTIVE_WG, o, n0, n1, n2, n1)
+            ext = B.scratch_extents(B.ADAPTIVE_WG, o, n0, n1, n2, 1)  # AdvectionSolver::window() == 1
             scratch = torch.empty(ext[0] * ext[1] * ext[2], dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
