@@ -36,17 +36,6 @@
 
 namespace bkma {
 
-template <int B, int E, class F> __device__ __forceinline__ void static_for(F &&f) {
-    if constexpr (B < E) {
-        f(std::integral_constant<int, B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
-constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
-constexpr int clcm(int a, int b) { return a / cgcd(a, b) * b; }
-constexpr int cmod(int a, int m) { return ((a % m) + m) % m; }
-
 // Native vector types for 4/8/16-byte global accesses.
 template <class T, int VEC> struct NativeVec { using type = T __attribute__((ext_vector_type(VEC))); };
 template <class T> struct NativeVec<T, 1> { using type = T; };

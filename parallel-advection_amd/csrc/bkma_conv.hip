@@ -4,11 +4,12 @@
 //   j in [0, L-K+1) overwrites x[b][oc][j]; x[b][oc][L-K+1 .. L) is kept.
 //
 // In-place safety: y[b][:, J] overwrites x[b][:, J], which every output
-// channel still needs.  The direct kernels give one work-group one batch
-// element and walk its columns in increasing tiles [j0, j0+TN): a tile reads
-// columns [j0, j0+TN+K-1) and writes [j0, j0+TN), so reads always run ahead
-// of writes.  The MFMA kernels process tiles in any order: the K-1 columns a
-// tile reads beyond its own are snapshotted by a pre-pass (conv_halo_kernel).
+// channel still needs.  Every kernel walks columns in increasing tiles
+// [j0, j0+TN): a tile reads columns [j0, j0+TN+K-1) and writes [j0, j0+TN),
+// so reads run ahead of writes.  The direct kernels give one work-group one
+// batch element; the MFMA kernels give one work-group a contiguous chunk of
+// the flattened (batch, tile) sequence, and the K-1 columns past a chunk's
+// last tile are snapshotted first (conv_halo_kernel).
 //
 // Kernels
 //  - conv_direct_kernel<T>: FP64 / FP32 FMA reference kernels (FP64 keeps the
@@ -27,6 +28,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -40,7 +42,9 @@ __device__ __forceinline__ __bf16 to_bf16(float f) { return (__bf16)f; }  // rou
 template <class T> __device__ __forceinline__ float io_to_f32(T v) { return (float)v; }
 template <> __device__ __forceinline__ float io_to_f32<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
 template <class T> __device__ __forceinline__ T f32_to_io(float v) { return (T)v; }
-template <> __device__ __forceinline__ __hip_bfloat16 f32_to_io<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
+template <> __device__ __forceinline__ __hip_bfloat16 f32_to_io<__hip_bfloat16>(float v) {
+    return __builtin_bit_cast(__hip_bfloat16, (__bf16)v);  // round to nearest even (v_cvt_pk_bf16_f32)
+}
 
 // ---------------------------------------------------------------------------
 // Direct kernels (FP64 / FP32)
@@ -127,22 +131,23 @@ __global__ void conv_pack_weights_kernel(const float *__restrict__ w, int C, int
     }
 }
 
-// In place without ordering between tiles: tile t of a batch element writes
-// columns [t*TN, t*TN + TN) and reads K-1 more; those belong to tile t+1, so
-// they are copied first, for every tile, into halo[b][t][ic][K-1].
+// In place: a work-group takes a contiguous chunk of tiles (flattened over
+// (b, t)) in increasing order, so a tile's K-1 columns beyond its own -- the
+// first columns of the next tile -- are read before that tile is written,
+// except for the last tile of a chunk, whose neighbour belongs to another
+// work-group.  Those K-1 columns are copied first into halo[chunk][ic][K-1].
 template <class TIO>
 __global__ void conv_halo_kernel(const TIO *__restrict__ x, TIO *__restrict__ halo, int C, int L, int K, int TN,
-                                 int tiles, i64 total) {
+                                 int tiles, int total_tiles, int chunk) {
+    const int c = blockIdx.x;
+    const int last = min((c + 1) * chunk, total_tiles) - 1;
+    if (last < c * chunk) return;  // empty chunk
+    const int b = last / tiles, t = last - b * tiles;
+    if (t >= tiles - 1) return;  // the row's last tile reads no neighbour
     const int H = K - 1;
-    for (i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (i64)gridDim.x * blockDim.x) {
-        const int kk = (int)(idx % H);
-        i64 r = idx / H;
-        const int ic = (int)(r % C);
-        r /= C;
-        const int t = (int)(r % (tiles - 1));
-        const i64 b = r / (tiles - 1);
-        const i64 j = (i64)(t + 1) * TN + kk;
-        halo[idx] = j < L ? x[(b * C + ic) * (i64)L + j] : (TIO)0.0f;
+    for (int e = threadIdx.x; e < C * H; e += blockDim.x) {
+        const int ic = e / H, kk = e - ic * H;
+        halo[(i64)c * C * H + e] = x[((i64)b * C + ic) * L + (i64)(t + 1) * TN + kk];
     }
 }
 
@@ -158,6 +163,21 @@ template <class T, int N> struct IoVec;
 template <> struct IoVec<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
 template <> struct IoVec<__hip_bfloat16, 4> { typedef unsigned short type __attribute__((ext_vector_type(4))); };
 
+// Columns j .. j+3 of a row of length L (j a multiple of 4), clamped into
+// the row and unmasked: the caller zeroes columns >= L when it consumes them,
+// so the loads stay in flight until then.
+template <class TIO, bool VL> __device__ __forceinline__ void load_quad_raw(const TIO *row, int j, int L, TIO (&out)[4]) {
+    if constexpr (VL) {
+        using V4 = typename IoVec<TIO, 4>::type;
+        const V4 v = *reinterpret_cast<const V4 *>(row + (j < L ? j : L - 4));
+#pragma unroll
+        for (int c = 0; c < 4; ++c) out[c] = io_elem<TIO>(v, c);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) out[c] = row[j + c < L ? j + c : L - 1];
+    }
+}
+
 // Persistent implicit-GEMM kernel.  A work-group (C/32 waves) owns all C
 // output channels of a tile of TN columns of one batch element; wave w keeps
 // the W fragments of channels 32w..32w+31 in registers.  Tiles are taken in
@@ -165,18 +185,19 @@ template <> struct IoVec<__hip_bfloat16, 4> { typedef unsigned short type __attr
 // are issued before the current tile's MFMAs and written to the other LDS
 // buffer after them, one barrier per tile.
 // LDS tile: Xs[jj][ic] bf16, row stride C + 8 (16-byte pad).
-template <class TIO, int NPROD, int C, int K, int TN>
+template <class TIO, int NPROD, int C, int K, int TN, bool VL>
 __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x, const TIO *__restrict__ halo,
                                                            const __bf16 *__restrict__ whi,
                                                            const __bf16 *__restrict__ wlo,
                                                            const float *__restrict__ bias, int L, int tiles,
-                                                           int total_tiles, int vec_ok) {
+                                                           int total_tiles, int chunk) {
     constexpr int NW = C / 32;
     constexpr int NT = 64 * NW;
     constexpr int KS = C * K / 16;
+    constexpr bool XLO = NPROD == 3 && !std::is_same<TIO, __hip_bfloat16>::value;  // bf16 input: x = xh exactly
     constexpr int JW = TN + K - 1;
     constexpr int RS = C + 8;
-    constexpr int NPART = NPROD == 3 ? 2 : 1;
+    constexpr int NPART = XLO ? 2 : 1;
     constexpr int BUF = JW * RS;  // bf16 elements of one part of one buffer
     constexpr int NSUB = TN / 32;
     constexpr int NQ = TN / 4;                // 4-column groups
@@ -184,8 +205,8 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
     static_assert(NTASK % NT == 0, "staging tasks must divide evenly");
     constexpr int TPT = NTASK / NT;
     constexpr int NH = C * (K - 1);
-    constexpr int HPT = (NH + NT - 1) / NT;
-    using V4 = typename IoVec<TIO, 4>::type;
+    static_assert(NH % NT == 0, "halo elements must divide evenly");
+    constexpr int HPT = NH / NT;
 
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __bf16 *xs = reinterpret_cast<__bf16 *>(lds_raw);  // [2][NPART][JW][RS]
@@ -193,8 +214,9 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
     const int lane = tid & 63, wv = tid >> 6;
     const int Lout = L - K + 1;
 
-    int tile = blockIdx.x;
-    if (tile >= total_tiles) return;  // uniform over the work-group
+    int tile = blockIdx.x * chunk;
+    const int tile_end = min(tile + chunk, total_tiles);
+    if (tile >= tile_end) return;  // empty chunk (uniform over the work-group)
 
     bf16x8_t wa[KS];
     bf16x8_t wb[NPROD == 3 ? KS : 1];
@@ -224,74 +246,66 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
             const int q = task % NQ, g = task / NQ;
             const int j = j0 + 4 * q;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const TIO *row = xb + (i64)(8 * g + i) * L;
-                if (vec_ok && j + 3 < L) {
-                    const V4 v = *reinterpret_cast<const V4 *>(row + j);
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) pf[u][i][c] = io_elem<TIO>(v, c);
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) pf[u][i][c] = j + c < L ? row[j + c] : (TIO)0.0f;
-                }
-            }
+            for (int i = 0; i < 8; ++i) load_quad_raw<TIO, VL>(xb + (i64)(8 * g + i) * L, j, L, pf[u][i]);
         }
+        const bool snap = tl == tile_end - 1 && t < tiles - 1;
 #pragma unroll
         for (int u = 0; u < HPT; ++u) {
             const int e = tid + u * NT;
-            if (e < NH) {
-                const int ic = e / (K - 1), kk = e - ic * (K - 1);
-                if (t < tiles - 1) {
-                    ph[u] = halo[(((i64)b * (tiles - 1) + t) * C + ic) * (K - 1) + kk];
-                } else {
-                    const int j = j0 + TN + kk;
-                    ph[u] = j < L ? xb[(i64)ic * L + j] : (TIO)0.0f;
-                }
-            }
+            const int ic = e / (K - 1), kk = e - ic * (K - 1);
+            const int j = j0 + TN + kk;
+            ph[u] = *(snap ? halo + (i64)blockIdx.x * NH + e : xb + (i64)ic * L + (j < L ? j : L - 1));
         }
     };
-    auto stage_tile = [&](int buf) __attribute__((always_inline)) {
+    // Converts the loaded tile tl (zero past L) into LDS buffer buf.
+    auto stage_tile = [&](int buf, int tl) __attribute__((always_inline)) {
         __bf16 *hi = xs + (buf * NPART) * BUF;
         __bf16 *lo = hi + BUF;
+        const int t = tl % tiles;
+        const int j0 = t * TN;
+        const bool snap = tl == tile_end - 1 && t < tiles - 1;
 #pragma unroll
         for (int u = 0; u < TPT; ++u) {
             const int task = tid + u * NT;
             const int q = task % NQ, g = task / NQ;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
+                const bool in = j0 + 4 * q + c < L;
                 bf16x8_t hv, lv;
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
-                    const float f = io_to_f32(pf[u][i][c]);
-                    const __bf16 h = to_bf16(f);
-                    hv[i] = h;
-                    if constexpr (NPROD == 3) lv[i] = to_bf16(f - (float)h);
+                    if constexpr (std::is_same<TIO, __hip_bfloat16>::value) {
+                        hv[i] = in ? __builtin_bit_cast(__bf16, pf[u][i][c]) : (__bf16)0.0f;
+                    } else {
+                        const float f = in ? io_to_f32(pf[u][i][c]) : 0.0f;
+                        const __bf16 h = to_bf16(f);
+                        hv[i] = h;
+                        if constexpr (XLO) lv[i] = to_bf16(f - (float)h);
+                    }
                 }
                 const int off = (4 * q + c) * RS + 8 * g;
                 *reinterpret_cast<bf16x8_t *>(hi + off) = hv;
-                if constexpr (NPROD == 3) *reinterpret_cast<bf16x8_t *>(lo + off) = lv;
+                if constexpr (XLO) *reinterpret_cast<bf16x8_t *>(lo + off) = lv;
             }
         }
 #pragma unroll
         for (int u = 0; u < HPT; ++u) {
             const int e = tid + u * NT;
-            if (e < NH) {
-                const int ic = e / (K - 1), kk = e - ic * (K - 1);
-                const float f = io_to_f32(ph[u]);
-                const __bf16 h = to_bf16(f);
-                hi[(TN + kk) * RS + ic] = h;
-                if constexpr (NPROD == 3) lo[(TN + kk) * RS + ic] = to_bf16(f - (float)h);
-            }
+            const int ic = e / (K - 1), kk = e - ic * (K - 1);
+            const float f = snap || j0 + TN + kk < L ? io_to_f32(ph[u]) : 0.0f;
+            const __bf16 h = to_bf16(f);
+            hi[(TN + kk) * RS + ic] = h;
+            if constexpr (XLO) lo[(TN + kk) * RS + ic] = to_bf16(f - (float)h);
         }
     };
 
     load_tile(tile);
-    stage_tile(0);
+    stage_tile(0, tile);
     __syncthreads();
     const int jl = lane % 32, kh = 8 * (lane / 32);
-    for (int it = 0; tile < total_tiles; ++it) {
-        const int next = tile + (int)gridDim.x;
-        if (next < total_tiles) load_tile(next);
+    for (int it = 0; tile < tile_end; ++it) {
+        const int next = tile + 1;
+        if (next < tile_end) load_tile(next);
         const __bf16 *hi = xs + ((it & 1) * NPART) * BUF;
         const __bf16 *lo = hi + BUF;
         f32x16_t acc[NSUB];
@@ -299,22 +313,29 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
         for (int n = 0; n < NSUB; ++n)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[n][r] = 0.0f;
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            const int r0 = 16 * s;  // compile-time after unrolling
-            const int k = r0 / C, ic0 = r0 - k * C + kh;
-#pragma unroll
-            for (int n = 0; n < NSUB; ++n) {
-                const int row = 32 * n + jl + k;
-                const bf16x8_t bh = *reinterpret_cast<const bf16x8_t *>(hi + row * RS + ic0);
-                acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[s], bh, acc[n], 0, 0, 0);
-                if constexpr (NPROD == 3) {
-                    const bf16x8_t bl = *reinterpret_cast<const bf16x8_t *>(lo + row * RS + ic0);
-                    acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[s], bl, acc[n], 0, 0, 0);
-                    acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wb[s], bh, acc[n], 0, 0, 0);
-                }
-            }
-        }
+        // B fragments are read PD steps ahead of their MFMA (a ring of PD+1),
+        // so LDS latency overlaps the MFMAs of the previous steps.
+        constexpr int NSTEP = KS * NSUB, PD = 2;
+        bf16x8_t bq[PD + 1], lq[XLO ? PD + 1 : 1];
+        auto bread = [&](auto ic_) __attribute__((always_inline)) {
+            constexpr int idx = decltype(ic_)::value;
+            constexpr int s = idx / NSUB, n = idx % NSUB;
+            constexpr int k = (16 * s) / C, icb = 16 * s - k * C;
+            const int off = (32 * n + jl + k) * RS + icb + kh;
+            bq[idx % (PD + 1)] = *reinterpret_cast<const bf16x8_t *>(hi + off);
+            if constexpr (XLO) lq[idx % (PD + 1)] = *reinterpret_cast<const bf16x8_t *>(lo + off);
+        };
+        static_for<0, PD>(bread);
+        static_for<0, NSTEP>([&](auto ic_) __attribute__((always_inline)) {
+            constexpr int idx = decltype(ic_)::value;
+            constexpr int s = idx / NSUB, n = idx % NSUB;
+            if constexpr (idx + PD < NSTEP) bread(std::integral_constant<int, idx + PD>{});
+            const bf16x8_t bh = bq[idx % (PD + 1)];
+            acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[s], bh, acc[n], 0, 0, 0);
+            if constexpr (XLO)
+                acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[s], lq[idx % (PD + 1)], acc[n], 0, 0, 0);
+            if constexpr (NPROD == 3) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wb[s], bh, acc[n], 0, 0, 0);
+        });
         // D[m][n]: lane l, register r -> m = 8*(r/4) + 4*(l/32) + r%4, n = l%32.
         {
             const int b = tile / tiles, t = tile - b * tiles;
@@ -331,7 +352,186 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
                 }
             }
         }
-        if (next < total_tiles) stage_tile((it + 1) & 1);
+        if (next < tile_end) stage_tile((it + 1) & 1, next);
+        __syncthreads();
+        tile = next;
+    }
+}
+
+// bf16 x bf16 with the k-steps of each 32-channel block split between two
+// waves (2*C/32 waves per work-group, two per SIMD): wave w keeps the W
+// fragments of channel block w % (C/32) for k-steps [h*KS/2, (h+1)*KS/2),
+// h = w / (C/32), i.e. half the registers of conv_mfma_kernel, so the next
+// tile's loads stay in flight during the MFMAs and a second wave hides
+// latencies.  The two partial sums meet in LDS: each wave of a pair adds the
+// partner's partial for half of the column sub-tiles and stores that half.
+template <class TIO, int C, int K, int TN, bool VL>
+__global__ __launch_bounds__(4 * C, 1) void conv_mfma_split_kernel(TIO *__restrict__ x, const TIO *__restrict__ halo,
+                                                                 const __bf16 *__restrict__ whi,
+                                                                 const float *__restrict__ bias, int L, int tiles,
+                                                                 int total_tiles, int chunk) {
+    constexpr int NB = C / 32;   // channel blocks
+    constexpr int NW = 2 * NB;   // waves
+    constexpr int NT = 64 * NW;
+    constexpr int KS = C * K / 16;
+    static_assert(KS % 2 == 0, "k-steps must split in halves");
+    constexpr int KH = KS / 2;
+    constexpr int JW = TN + K - 1;
+    constexpr int RS = C + 8;
+    constexpr int BUF = JW * RS;
+    constexpr int NSUB = TN / 32;
+    static_assert(NSUB % 2 == 0, "column sub-tiles must split in halves");
+    constexpr int NQ = TN / 4;
+    constexpr int NTASK = (C / 8) * NQ;
+    static_assert(NTASK % NT == 0, "staging tasks must divide evenly");
+    constexpr int TPT = NTASK / NT;
+    constexpr int NH = C * (K - 1);
+    constexpr int HPT = (NH + NT - 1) / NT;
+
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __bf16 *xs = reinterpret_cast<__bf16 *>(lds_raw);                       // [2][JW][RS]
+    float *red = reinterpret_cast<float *>(lds_raw + 2 * BUF * sizeof(__bf16));  // [NB][2][NSUB/2][16][64]
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int ob = wv % NB, h = wv / NB;
+    const int Lout = L - K + 1;
+
+    int tile = blockIdx.x * chunk;
+    const int tile_end = min(tile + chunk, total_tiles);
+    if (tile >= tile_end) return;  // empty chunk (uniform over the work-group)
+
+    bf16x8_t wa[KH];
+    {
+        const bf16x8_t *src = reinterpret_cast<const bf16x8_t *>(whi) + ((i64)ob * KS + h * KH) * 64 + lane;
+#pragma unroll
+        for (int s = 0; s < KH; ++s) wa[s] = src[s * 64];
+    }
+
+    TIO pf[TPT][8][4];
+    TIO ph[HPT];
+    auto load_tile = [&](int tl) __attribute__((always_inline)) {
+        const int b = tl / tiles, t = tl - b * tiles;
+        const int j0 = t * TN;
+        const TIO *xb = x + (i64)b * C * L;
+#pragma unroll
+        for (int u = 0; u < TPT; ++u) {
+            const int task = tid + u * NT;
+            const int q = task % NQ, g = task / NQ;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) load_quad_raw<TIO, VL>(xb + (i64)(8 * g + i) * L, j0 + 4 * q, L, pf[u][i]);
+        }
+        const bool snap = tl == tile_end - 1 && t < tiles - 1;
+#pragma unroll
+        for (int u = 0; u < HPT; ++u) {
+            const int e = min(tid + u * NT, NH - 1);  // threads past NH repeat the last element
+            const int ic = e / (K - 1), kk = e - ic * (K - 1);
+            const int j = j0 + TN + kk;
+            ph[u] = *(snap ? halo + (i64)blockIdx.x * NH + e : xb + (i64)ic * L + (j < L ? j : L - 1));
+        }
+    };
+    // Converts the loaded tile tl (zero past L) into LDS buffer buf.
+    auto stage_tile = [&](int buf, int tl) __attribute__((always_inline)) {
+        __bf16 *hi = xs + buf * BUF;
+        const int b = tl / tiles, t = tl - b * tiles;
+        const int j0 = t * TN;
+        const bool snap = tl == tile_end - 1 && t < tiles - 1;
+#pragma unroll
+        for (int u = 0; u < TPT; ++u) {
+            const int task = tid + u * NT;
+            const int q = task % NQ, g = task / NQ;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const bool in = j0 + 4 * q + c < L;
+                bf16x8_t hv;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    __bf16 hb;
+                    if constexpr (std::is_same<TIO, __hip_bfloat16>::value)
+                        hb = __builtin_bit_cast(__bf16, pf[u][i][c]);
+                    else
+                        hb = to_bf16(io_to_f32(pf[u][i][c]));
+                    hv[i] = in ? hb : (__bf16)0.0f;
+                }
+                *reinterpret_cast<bf16x8_t *>(hi + (4 * q + c) * RS + 8 * g) = hv;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < HPT; ++u) {
+            const int e = tid + u * NT;
+            const int ic = e / (K - 1), kk = e - ic * (K - 1);
+            const bool in = snap || j0 + TN + kk < L;
+            if (e < NH) hi[(TN + kk) * RS + ic] = in ? to_bf16(io_to_f32(ph[u])) : (__bf16)0.0f;
+        }
+    };
+
+    load_tile(tile);
+    stage_tile(0, tile);
+    __syncthreads();
+    const int jl = lane % 32, kh = 8 * (lane / 32);
+    float *red_out = red + ((ob * 2 + (1 - h)) * (NSUB / 2)) * 16 * 64;  // the partner's half
+    const float *red_in = red + ((ob * 2 + h) * (NSUB / 2)) * 16 * 64;   // this wave's half
+    for (int it = 0; tile < tile_end; ++it) {
+        const int next = tile + 1;
+        if (next < tile_end) load_tile(next);
+        const __bf16 *hi = xs + (it & 1) * BUF;
+        f32x16_t acc[NSUB];
+#pragma unroll
+        for (int n = 0; n < NSUB; ++n)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[n][r] = 0.0f;
+        // The k-loop is instantiated for each half (h is wave-uniform), so
+        // every LDS offset is a compile-time constant.
+        auto kloop = [&](auto hc_) __attribute__((always_inline)) {
+            constexpr int H = decltype(hc_)::value;
+            constexpr int NSTEP = KH * NSUB, PD = 2;
+            const __bf16 *base = hi + jl * RS + kh;
+            bf16x8_t bq[PD + 1];
+            auto bread = [&](auto ic_) __attribute__((always_inline)) {
+                constexpr int idx = decltype(ic_)::value;
+                constexpr int sl = idx / NSUB, n = idx % NSUB;
+                constexpr int sg = H * KH + sl;  // global k-step
+                constexpr int k = (16 * sg) / C, icb = 16 * sg - k * C;
+                bq[idx % (PD + 1)] = *reinterpret_cast<const bf16x8_t *>(base + (32 * n + k) * RS + icb);
+            };
+            static_for<0, PD>(bread);
+            static_for<0, NSTEP>([&](auto ic_) __attribute__((always_inline)) {
+                constexpr int idx = decltype(ic_)::value;
+                constexpr int sl = idx / NSUB, n = idx % NSUB;
+                if constexpr (idx + PD < NSTEP) bread(std::integral_constant<int, idx + PD>{});
+                acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[sl], bq[idx % (PD + 1)], acc[n], 0, 0, 0);
+            });
+        };
+        if (h == 0)
+            kloop(std::integral_constant<int, 0>{});
+        else
+            kloop(std::integral_constant<int, 1>{});
+        // Exchange partial sums: h = 0 keeps sub-tiles [0, NSUB/2), h = 1 the rest.
+#pragma unroll
+        for (int n2 = 0; n2 < NSUB / 2; ++n2) {
+            // the partner's sub-tile (n2 for h = 1, NSUB/2 + n2 for h = 0); selects keep acc in registers
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                red_out[(n2 * 16 + r) * 64 + lane] = h ? acc[n2][r] : acc[NSUB / 2 + n2][r];
+        }
+        __syncthreads();
+        {
+            const int b = tile / tiles, t = tile - b * tiles;
+            TIO *xb = x + (i64)b * C * L;
+#pragma unroll
+            for (int n2 = 0; n2 < NSUB / 2; ++n2) {
+                const int n = h * (NSUB / 2) + n2;
+                const int j = t * TN + 32 * n + jl;
+                // D[m][n]: lane l, register r -> m = 8*(r/4) + 4*(l/32) + r%4, n = l%32.
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const float mine = h ? acc[NSUB / 2 + n2][r] : acc[n2][r];
+                    const float v = mine + red_in[(n2 * 16 + r) * 64 + lane];
+                    const int oc = 32 * ob + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+                    if (j < Lout) xb[(i64)oc * L + j] = f32_to_io<TIO>(v + bias[oc]);
+                }
+            }
+        }
+        if (next < tile_end) stage_tile((it + 1) & 1, next);
         __syncthreads();
         tile = next;
     }
@@ -416,12 +616,12 @@ __global__ void conv_fp8_pack_weights_kernel(const float *__restrict__ w, const 
 // in LDS as FP8 e4m3 (x / xscale), channel-last, double-buffered; the next
 // tile's global loads, conversion and LDS stores are spread over the k-loop
 // of the current one.  y = acc * wscale[oc] * xscale + bias[oc].
-template <class TIO, int C, int K, int TN>
+template <class TIO, int C, int K, int TN, bool VL>
 __global__ __launch_bounds__(2 * C, 1) void conv_fp8_kernel(TIO *__restrict__ x, const TIO *__restrict__ halo,
                                                           const i32x8_t *__restrict__ wq,
                                                           const float *__restrict__ wscale,
                                                           const float *__restrict__ bias, float xscale, int L,
-                                                          int tiles, int total_tiles, int vec_ok) {
+                                                          int tiles, int total_tiles, int chunk) {
     constexpr int NW = C / 32;
     constexpr int NT = 64 * NW;
     constexpr int KS = C * K / 64;
@@ -436,8 +636,8 @@ __global__ __launch_bounds__(2 * C, 1) void conv_fp8_kernel(TIO *__restrict__ x,
     static_assert(KS % TPT == 0, "k-steps must split evenly over the staging tasks");
     constexpr int PH = KS / TPT;  // k-steps per staging task
     constexpr int NH = C * (K - 1);
-    constexpr int HPT = (NH + NT - 1) / NT;
-    using V4 = typename IoVec<TIO, 4>::type;
+    static_assert(NH % NT == 0, "halo elements must divide evenly");
+    constexpr int HPT = NH / NT;
 
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int tid = threadIdx.x;
@@ -445,16 +645,10 @@ __global__ __launch_bounds__(2 * C, 1) void conv_fp8_kernel(TIO *__restrict__ x,
     const int Lout = L - K + 1;
     const float inv_xs = 1.0f / xscale;
 
-    int tile = blockIdx.x;
-    if (tile >= total_tiles) return;  // uniform over the work-group
+    int tile = blockIdx.x * chunk;
+    const int tile_end = min(tile + chunk, total_tiles);
+    if (tile >= tile_end) return;  // empty chunk (uniform over the work-group)
 
-    float post[16], bias_r[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int oc = 32 * wv + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
-        post[r] = wscale[oc] * xscale;
-        bias_r[r] = bias[oc];
-    }
     const i32x8_t *wsrc = wq + ((i64)wv * KS) * 64 + lane;
 
     TIO pf[8][4];
@@ -466,73 +660,61 @@ __global__ __launch_bounds__(2 * C, 1) void conv_fp8_kernel(TIO *__restrict__ x,
         const int j = t * TN + 4 * q;
         const TIO *xb = x + (i64)b * C * L;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const TIO *row = xb + (i64)(8 * g + i) * L;
-            if (vec_ok && j + 3 < L) {
-                const V4 v = *reinterpret_cast<const V4 *>(row + j);
-#pragma unroll
-                for (int c = 0; c < 4; ++c) pf[i][c] = io_elem<TIO>(v, c);
-            } else {
-#pragma unroll
-                for (int c = 0; c < 4; ++c) pf[i][c] = j + c < L ? row[j + c] : (TIO)0.0f;
-            }
-        }
+        for (int i = 0; i < 8; ++i) load_quad_raw<TIO, VL>(xb + (i64)(8 * g + i) * L, j, L, pf[i]);
     };
-    auto store_task = [&](int buf, int u) __attribute__((always_inline)) {
+    auto store_task = [&](int buf, int tl, int u) __attribute__((always_inline)) {
         unsigned char *xs = lds_raw + buf * BUF;
         const int task = tid + u * NT;
         const int q = task % NQ, g = task / NQ;
+        const int j = (tl % tiles) * TN + 4 * q;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
+            const float sc = j + c < L ? inv_xs : 0.0f;  // zero past L
             i32x2_t v;
-            v[0] = (pack_fp8x2(io_to_f32(pf[0][c]) * inv_xs, io_to_f32(pf[1][c]) * inv_xs) & 0xffff) |
-                   (pack_fp8x2(io_to_f32(pf[2][c]) * inv_xs, io_to_f32(pf[3][c]) * inv_xs) << 16);
-            v[1] = (pack_fp8x2(io_to_f32(pf[4][c]) * inv_xs, io_to_f32(pf[5][c]) * inv_xs) & 0xffff) |
-                   (pack_fp8x2(io_to_f32(pf[6][c]) * inv_xs, io_to_f32(pf[7][c]) * inv_xs) << 16);
+            v[0] = (pack_fp8x2(io_to_f32(pf[0][c]) * sc, io_to_f32(pf[1][c]) * sc) & 0xffff) |
+                   (pack_fp8x2(io_to_f32(pf[2][c]) * sc, io_to_f32(pf[3][c]) * sc) << 16);
+            v[1] = (pack_fp8x2(io_to_f32(pf[4][c]) * sc, io_to_f32(pf[5][c]) * sc) & 0xffff) |
+                   (pack_fp8x2(io_to_f32(pf[6][c]) * sc, io_to_f32(pf[7][c]) * sc) << 16);
             *reinterpret_cast<i32x2_t *>(xs + (4 * q + c) * RS + 8 * g) = v;
         }
     };
     auto load_halo = [&](int tl) __attribute__((always_inline)) {
         const int b = tl / tiles, t = tl - b * tiles;
         const TIO *xb = x + (i64)b * C * L;
+        const bool snap = tl == tile_end - 1 && t < tiles - 1;
 #pragma unroll
         for (int u = 0; u < HPT; ++u) {
             const int e = tid + u * NT;
-            if (e < NH) {
-                const int ic = e / (K - 1), kk = e - ic * (K - 1);
-                if (t < tiles - 1) {
-                    ph[u] = halo[(((i64)b * (tiles - 1) + t) * C + ic) * (K - 1) + kk];
-                } else {
-                    const int j = t * TN + TN + kk;
-                    ph[u] = j < L ? xb[(i64)ic * L + j] : (TIO)0.0f;
-                }
-            }
+            const int ic = e / (K - 1), kk = e - ic * (K - 1);
+            const int j = t * TN + TN + kk;
+            ph[u] = *(snap ? halo + (i64)blockIdx.x * NH + e : xb + (i64)ic * L + (j < L ? j : L - 1));
         }
     };
-    auto store_halo = [&](int buf) __attribute__((always_inline)) {
+    auto store_halo = [&](int buf, int tl) __attribute__((always_inline)) {
         unsigned char *xs = lds_raw + buf * BUF;
+        const int t = tl % tiles;
+        const bool snap = tl == tile_end - 1 && t < tiles - 1;
 #pragma unroll
         for (int u = 0; u < HPT; ++u) {
             const int e = tid + u * NT;
-            if (e < NH) {
-                const int ic = e / (K - 1), kk = e - ic * (K - 1);
-                xs[(TN + kk) * RS + ic] = (unsigned char)(pack_fp8x2(io_to_f32(ph[u]) * inv_xs, 0.0f) & 0xff);
-            }
+            const int ic = e / (K - 1), kk = e - ic * (K - 1);
+            const float sc = snap || t * TN + TN + kk < L ? inv_xs : 0.0f;
+            xs[(TN + kk) * RS + ic] = (unsigned char)(pack_fp8x2(io_to_f32(ph[u]) * sc, 0.0f) & 0xff);
         }
     };
 
 #pragma unroll
     for (int u = 0; u < TPT; ++u) {
         load_task(tile, u);
-        store_task(0, u);
+        store_task(0, tile, u);
     }
     load_halo(tile);
-    store_halo(0);
+    store_halo(0, tile);
     __syncthreads();
     const int jl = lane % 32, kh = 32 * (lane / 32);
-    for (int it = 0; tile < total_tiles; ++it) {
-        const int next = tile + (int)gridDim.x;
-        const bool has_next = next < total_tiles;
+    for (int it = 0; tile < tile_end; ++it) {
+        const int next = tile + 1;
+        const bool has_next = next < tile_end;
         const unsigned char *xs = lds_raw + (it & 1) * BUF;
         f32x16_t acc[NSUB];
 #pragma unroll
@@ -543,7 +725,7 @@ __global__ __launch_bounds__(2 * C, 1) void conv_fp8_kernel(TIO *__restrict__ x,
         i32x8_t wcur = wsrc[0];
         for (int u = 0; u < TPT; ++u) {
             if (has_next) load_task(next, u);
-#pragma unroll 2
+#pragma unroll 1
             for (int ss = 0; ss < PH; ++ss) {
                 const int s = u * PH + ss;
                 const i32x8_t wnext = wsrc[((s + 1) % KS) * 64];  // next k-step (wraps to the next tile's first)
@@ -560,9 +742,9 @@ __global__ __launch_bounds__(2 * C, 1) void conv_fp8_kernel(TIO *__restrict__ x,
                 }
                 wcur = wnext;
             }
-            if (has_next) store_task((it + 1) & 1, u);
+            if (has_next) store_task((it + 1) & 1, next, u);
         }
-        if (has_next) store_halo((it + 1) & 1);
+        if (has_next) store_halo((it + 1) & 1, next);
         {
             const int b = tile / tiles, t = tile - b * tiles;
             TIO *xb = x + (i64)b * C * L;
@@ -572,8 +754,9 @@ __global__ __launch_bounds__(2 * C, 1) void conv_fp8_kernel(TIO *__restrict__ x,
                 if (j < Lout) {
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
+                        // scales and bias re-read here (L1-resident) rather than held in registers
                         const int oc = 32 * wv + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
-                        xb[(i64)oc * L + j] = f32_to_io<TIO>(acc[n][r] * post[r] + bias_r[r]);
+                        xb[(i64)oc * L + j] = f32_to_io<TIO>(acc[n][r] * (wscale[oc] * xscale) + bias[oc]);
                     }
                 }
             }
@@ -638,22 +821,44 @@ template <class Dst, class Src> __global__ void convert_kernel(Dst *dst, const S
 
 using MfmaLaunch = void (*)(void *x, const ConvPlan &p, hipStream_t s);
 
-template <class TIO> void launch_halo(void *x, const ConvPlan &p, int C, int K, int TN, int tiles, hipStream_t s);
+template <class TIO>
+void launch_halo(void *x, const ConvPlan &p, int C, int K, int TN, int tiles, int total, int *grid, int *chunk,
+                 hipStream_t s);
 
 template <class TIO, int NPROD, int C, int K, int TN> void launch_mfma(void *x, const ConvPlan &p, hipStream_t s) {
     const int tiles = (int)ceil_div((u64)(p.L - p.K + 1), (u64)TN);
     const int total = (int)(p.B * tiles);
-    launch_halo<TIO>(x, p, C, K, TN, tiles, s);
-    auto k = conv_mfma_kernel<TIO, NPROD, C, K, TN>;
+    int grid = 0, chunk = 0;
+    launch_halo<TIO>(x, p, C, K, TN, tiles, total, &grid, &chunk, s);
+    auto k = p.L % 4 == 0 ? conv_mfma_kernel<TIO, NPROD, C, K, TN, true> : conv_mfma_kernel<TIO, NPROD, C, K, TN, false>;
     (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds_bytes);
-    const unsigned grid = (unsigned)std::min<i64>(total, p.resident_groups);
     hipLaunchKernelGGL(k, dim3(grid), dim3(2 * C), (size_t)p.lds_bytes, s, reinterpret_cast<TIO *>(x),
                        reinterpret_cast<const TIO *>(p.halo), (const __bf16 *)p.w, (const __bf16 *)p.wlo,
-                       (const float *)p.bias, (int)p.L, tiles, total, (int)(p.L % 4 == 0));
+                       (const float *)p.bias, (int)p.L, tiles, total, chunk);
+}
+
+template <class TIO, int C, int K, int TN> void launch_mfma_split(void *x, const ConvPlan &p, hipStream_t s) {
+    const int tiles = (int)ceil_div((u64)(p.L - p.K + 1), (u64)TN);
+    const int total = (int)(p.B * tiles);
+    int grid = 0, chunk = 0;
+    launch_halo<TIO>(x, p, C, K, TN, tiles, total, &grid, &chunk, s);
+    auto k = p.L % 4 == 0 ? conv_mfma_split_kernel<TIO, C, K, TN, true> : conv_mfma_split_kernel<TIO, C, K, TN, false>;
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds_bytes);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(4 * C), (size_t)p.lds_bytes, s, reinterpret_cast<TIO *>(x),
+                       reinterpret_cast<const TIO *>(p.halo), (const __bf16 *)p.w, (const float *)p.bias, (int)p.L,
+                       tiles, total, chunk);
+}
+
+template <class TIO, int C, int K, int TN> int occupancy_mfma_split(u64 lds_bytes) {
+    auto k = conv_mfma_split_kernel<TIO, C, K, TN, true>;
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 4 * C, (size_t)lds_bytes) != hipSuccess) return 0;
+    return blocks;
 }
 
 template <class TIO, int NPROD, int C, int K, int TN> int occupancy_mfma(u64 lds_bytes) {
-    auto k = conv_mfma_kernel<TIO, NPROD, C, K, TN>;
+    auto k = conv_mfma_kernel<TIO, NPROD, C, K, TN, true>;
     (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 2 * C, (size_t)lds_bytes) != hipSuccess) return 0;
@@ -665,39 +870,54 @@ struct MfmaVariant {
     int nprod, C, K, TN;
     MfmaLaunch fn;
     int (*occ)(u64 lds_bytes);
+    int split;  // conv_mfma_split_kernel (two waves per channel block)
 };
 
-#define MV(TIO, IO, NP, C, K, TN) {IO, NP, C, K, TN, &launch_mfma<TIO, NP, C, K, TN>, &occupancy_mfma<TIO, NP, C, K, TN>}
+#define MV(TIO, IO, NP, C, K, TN) \
+    {IO, NP, C, K, TN, &launch_mfma<TIO, NP, C, K, TN>, &occupancy_mfma<TIO, NP, C, K, TN>, 0}
+#define MS(TIO, IO, C, K, TN) \
+    {IO, 1, C, K, TN, &launch_mfma_split<TIO, C, K, TN>, &occupancy_mfma_split<TIO, C, K, TN>, 1}
+// First match wins.
 const MfmaVariant kMfmaVariants[] = {
+#ifndef BKMA_CONV_NO_SPLIT
+    MS(float, BKMA_F32, 128, 5, 128),              MS(__hip_bfloat16, BKMA_BF16, 128, 5, 128),
+    MS(float, BKMA_F32, 128, 3, 128),              MS(float, BKMA_F32, 64, 5, 128),
+#endif
     MV(float, BKMA_F32, 1, 128, 5, 128),           MV(float, BKMA_F32, 3, 128, 5, 64),
     MV(__hip_bfloat16, BKMA_BF16, 1, 128, 5, 128), MV(__hip_bfloat16, BKMA_BF16, 3, 128, 5, 64),
     MV(float, BKMA_F32, 1, 128, 3, 128),           MV(float, BKMA_F32, 3, 128, 3, 64),
     MV(float, BKMA_F32, 1, 64, 5, 128),            MV(float, BKMA_F32, 3, 64, 5, 64),
 };
 #undef MV
+#undef MS
 
-template <class TIO> void launch_halo(void *x, const ConvPlan &p, int C, int K, int TN, int tiles, hipStream_t s) {
-    if (tiles <= 1) return;
-    const i64 nh = p.B * (tiles - 1) * C * (K - 1);
-    const unsigned g = (unsigned)std::min<i64>(ceil_div((u64)nh, 256ull), 65536);
-    hipLaunchKernelGGL(conv_halo_kernel<TIO>, dim3(g), dim3(256), 0, s, reinterpret_cast<const TIO *>(x),
-                       reinterpret_cast<TIO *>(p.halo), C, (int)p.L, K, TN, tiles, nh);
+// Grid (one work-group per chunk of consecutive tiles) and the halo copies
+// of the chunks' last tiles.
+template <class TIO>
+void launch_halo(void *x, const ConvPlan &p, int C, int K, int TN, int tiles, int total, int *grid, int *chunk,
+                 hipStream_t s) {
+    const int g = (int)std::max<i64>(1, std::min<i64>(total, p.resident_groups));
+    *chunk = (int)ceil_div((u64)total, (u64)g);
+    *grid = (int)ceil_div((u64)total, (u64)*chunk);
+    if (tiles > 1)
+        hipLaunchKernelGGL(conv_halo_kernel<TIO>, dim3(*grid), dim3(256), 0, s, reinterpret_cast<const TIO *>(x),
+                           reinterpret_cast<TIO *>(p.halo), C, (int)p.L, K, TN, tiles, total, *chunk);
 }
 
 template <class TIO, int C, int K, int TN> void launch_fp8(void *x, const ConvPlan &p, hipStream_t s) {
     const int tiles = (int)ceil_div((u64)(p.L - p.K + 1), (u64)TN);
     const int total = (int)(p.B * tiles);
-    launch_halo<TIO>(x, p, C, K, TN, tiles, s);
-    auto k = conv_fp8_kernel<TIO, C, K, TN>;
+    int grid = 0, chunk = 0;
+    launch_halo<TIO>(x, p, C, K, TN, tiles, total, &grid, &chunk, s);
+    auto k = p.L % 4 == 0 ? conv_fp8_kernel<TIO, C, K, TN, true> : conv_fp8_kernel<TIO, C, K, TN, false>;
     (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds_bytes);
-    const unsigned grid = (unsigned)std::min<i64>(total, p.resident_groups);
     hipLaunchKernelGGL(k, dim3(grid), dim3(2 * C), (size_t)p.lds_bytes, s, reinterpret_cast<TIO *>(x),
                        reinterpret_cast<const TIO *>(p.halo), (const i32x8_t *)p.w, (const float *)p.wscale,
-                       (const float *)p.bias, p.xscale, (int)p.L, tiles, total, (int)(p.L % 4 == 0));
+                       (const float *)p.bias, p.xscale, (int)p.L, tiles, total, chunk);
 }
 
 template <class TIO, int C, int K, int TN> int occupancy_fp8(u64 lds_bytes) {
-    auto k = conv_fp8_kernel<TIO, C, K, TN>;
+    auto k = conv_fp8_kernel<TIO, C, K, TN, true>;
     (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 2 * C, (size_t)lds_bytes) != hipSuccess) return 0;
@@ -783,19 +1003,19 @@ extern "C" bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype i
                            (__bf16 *)p->w, (__bf16 *)p->wlo);
         if (hipMemcpyAsync(p->bias, bias, C * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
             return bail(fail(BKMA_ERR_DEVICE, "conv1d plan: bias copy"));
-        p->lds_bytes = 2ull * (u64)(mv->TN + K - 1) * (u64)(C + 8) * 2 * (nprod == 3 ? 2 : 1);
+        const bool xlo = nprod == 3 && io_dtype != BKMA_BF16;
+        p->lds_bytes = 2ull * (u64)(mv->TN + K - 1) * (u64)(C + 8) * 2 * (xlo ? 2 : 1);
+        if (mv->split) p->lds_bytes += (u64)(C / 32) * (u64)(mv->TN / 32) * 16 * 64 * 4;  // partial-sum exchange
         if (p->lds_bytes > lds_max) return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: LDS tile too large"));
-        const u64 tiles = ceil_div(L - K + 1, (u64)mv->TN);
-        const u64 esz = io_dtype == BKMA_BF16 ? 2 : 4;
-        const u64 hbytes = std::max<u64>(B * (tiles - 1) * C * (K - 1) * esz, 16);
-        if (hipMalloc(&p->halo, hbytes) != hipSuccess)
-            return bail(fail(BKMA_ERR_OUT_OF_MEMORY, "conv1d plan: halo buffer"));
         int dev = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return bail(fail(BKMA_ERR_DEVICE, "conv1d plan: device query"));
         p->resident_groups = (i64)std::max(1, mv->occ(p->lds_bytes)) * cus;
-        p->name = std::string("conv_mfma_bf16") + (nprod == 3 ? "x3" : "") + "<C" + std::to_string(C) + ",K" +
+        const u64 esz = io_dtype == BKMA_BF16 ? 2 : 4;
+        if (hipMalloc(&p->halo, std::max<u64>((u64)p->resident_groups * C * (K - 1) * esz, 16)) != hipSuccess)
+            return bail(fail(BKMA_ERR_OUT_OF_MEMORY, "conv1d plan: halo buffer"));
+        p->name = std::string(mv->split ? "conv_mfma_split_bf16" : "conv_mfma_bf16") + (nprod == 3 ? "x3" : "") + "<C" + std::to_string(C) + ",K" +
                   std::to_string(K) + ",TN" + std::to_string(mv->TN) + (io_dtype == BKMA_BF16 ? ",io_bf16" : ",io_f32") +
                   ">";
     } else if (mode == BKMA_CONV_FP8) {
@@ -815,15 +1035,14 @@ extern "C" bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype i
             return bail(fail(BKMA_ERR_DEVICE, "conv1d plan: bias copy"));
         p->lds_bytes = 2ull * (u64)(mv->TN + K - 1) * (u64)(C + 16);
         if (p->lds_bytes > lds_max) return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: LDS tile too large"));
-        const u64 tiles = ceil_div(L - K + 1, (u64)mv->TN);
-        const u64 esz = io_dtype == BKMA_BF16 ? 2 : 4;
-        if (hipMalloc(&p->halo, std::max<u64>(B * (tiles - 1) * C * (K - 1) * esz, 16)) != hipSuccess)
-            return bail(fail(BKMA_ERR_OUT_OF_MEMORY, "conv1d plan: halo buffer"));
         int dev = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return bail(fail(BKMA_ERR_DEVICE, "conv1d plan: device query"));
         p->resident_groups = (i64)std::max(1, mv->occ(p->lds_bytes)) * cus;
+        const u64 esz = io_dtype == BKMA_BF16 ? 2 : 4;
+        if (hipMalloc(&p->halo, std::max<u64>((u64)p->resident_groups * C * (K - 1) * esz, 16)) != hipSuccess)
+            return bail(fail(BKMA_ERR_OUT_OF_MEMORY, "conv1d plan: halo buffer"));
         p->name = std::string("conv_fp8_e4m3<C") + std::to_string(C) + ",K" + std::to_string(K) + ",TN" +
                   std::to_string(mv->TN) + (io_dtype == BKMA_BF16 ? ",io_bf16" : ",io_f32") + ">";
     } else {

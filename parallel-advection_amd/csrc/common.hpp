@@ -1,6 +1,8 @@
 // Shared host/device helpers of the gfx950 BKMA library.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
@@ -71,6 +73,18 @@ template <class T> __device__ __forceinline__ T from_f64(double v) { return (T)v
 // ---------------------------------------------------------------------------
 // Advection parameters as the kernels see them (subset of bkma_adv_params).
 // ---------------------------------------------------------------------------
+// Compile-time loop: f(integral_constant<int, I>) for I = B .. E-1.
+template <int B, int E, class F> __device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
+constexpr int clcm(int a, int b) { return a / cgcd(a, b) * b; }
+constexpr int cmod(int a, int m) { return ((a % m) + m) % m; }
+
 struct AdvDev {
     i64 n0, n1, n2;
     double dt, dx, dvx, inv_dx;
