@@ -41,7 +41,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "adv_column_stream_f32_2048x1024x512.traffic.json"))
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "adv_column_stream_f32_2048x1024x512.traffic.txt"),
+                    help="JSON summary of the rocprofv3 FETCH_SIZE/WRITE_SIZE passes (hbm_bytes_per_launch)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (3 launches), for scaling rocprofv3 byte counters")
     ap.add_argument("--extra", default="",
@@ -72,13 +73,15 @@ def cpu_baseline(target_s: float):
                       f"oracle/bkma_oracle.c (OpenMP over lines), {el:.2f} s after one untimed step"}
 
 
-def load_traffic(path, bytes_alg):
+def load_traffic(path, plan_name):
+    """HBM bytes per launch from the committed rocprofv3 counter summary,
+    if it was taken for the same plan (else None)."""
     try:
         with open(path) as f:
             t = json.load(f)
-        return t.get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         return None
+    return t.get("hbm_bytes_per_launch") if t.get("plan") == plan_name else None
 
 
 def main():
@@ -194,7 +197,7 @@ def main():
                    "n0": n0, "n1": N1, "n2": N2, "dt": DT, "kernel": name32},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(args.traffic, bytes_alg),
+                     "traffic": load_traffic(args.traffic, name32) if n0 == N0 else None,
                      "kernel_avg_ms": round(kern_ms32, 5), "algorithmic_bytes_per_launch": bytes_alg,
                      "steps_per_launch": spp,
                      "per_step_equivalent": {"GBps": round(per_step_gbs, 1),

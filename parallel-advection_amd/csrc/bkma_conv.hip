@@ -358,185 +358,6 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
     }
 }
 
-// bf16 x bf16 with the k-steps of each 32-channel block split between two
-// waves (2*C/32 waves per work-group, two per SIMD): wave w keeps the W
-// fragments of channel block w % (C/32) for k-steps [h*KS/2, (h+1)*KS/2),
-// h = w / (C/32), i.e. half the registers of conv_mfma_kernel, so the next
-// tile's loads stay in flight during the MFMAs and a second wave hides
-// latencies.  The two partial sums meet in LDS: each wave of a pair adds the
-// partner's partial for half of the column sub-tiles and stores that half.
-template <class TIO, int C, int K, int TN, bool VL>
-__global__ __launch_bounds__(4 * C, 1) void conv_mfma_split_kernel(TIO *__restrict__ x, const TIO *__restrict__ halo,
-                                                                 const __bf16 *__restrict__ whi,
-                                                                 const float *__restrict__ bias, int L, int tiles,
-                                                                 int total_tiles, int chunk) {
-    constexpr int NB = C / 32;   // channel blocks
-    constexpr int NW = 2 * NB;   // waves
-    constexpr int NT = 64 * NW;
-    constexpr int KS = C * K / 16;
-    static_assert(KS % 2 == 0, "k-steps must split in halves");
-    constexpr int KH = KS / 2;
-    constexpr int JW = TN + K - 1;
-    constexpr int RS = C + 8;
-    constexpr int BUF = JW * RS;
-    constexpr int NSUB = TN / 32;
-    static_assert(NSUB % 2 == 0, "column sub-tiles must split in halves");
-    constexpr int NQ = TN / 4;
-    constexpr int NTASK = (C / 8) * NQ;
-    static_assert(NTASK % NT == 0, "staging tasks must divide evenly");
-    constexpr int TPT = NTASK / NT;
-    constexpr int NH = C * (K - 1);
-    constexpr int HPT = (NH + NT - 1) / NT;
-
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    __bf16 *xs = reinterpret_cast<__bf16 *>(lds_raw);                       // [2][JW][RS]
-    float *red = reinterpret_cast<float *>(lds_raw + 2 * BUF * sizeof(__bf16));  // [NB][2][NSUB/2][16][64]
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wv = tid >> 6;
-    const int ob = wv % NB, h = wv / NB;
-    const int Lout = L - K + 1;
-
-    int tile = blockIdx.x * chunk;
-    const int tile_end = min(tile + chunk, total_tiles);
-    if (tile >= tile_end) return;  // empty chunk (uniform over the work-group)
-
-    bf16x8_t wa[KH];
-    {
-        const bf16x8_t *src = reinterpret_cast<const bf16x8_t *>(whi) + ((i64)ob * KS + h * KH) * 64 + lane;
-#pragma unroll
-        for (int s = 0; s < KH; ++s) wa[s] = src[s * 64];
-    }
-
-    TIO pf[TPT][8][4];
-    TIO ph[HPT];
-    auto load_tile = [&](int tl) __attribute__((always_inline)) {
-        const int b = tl / tiles, t = tl - b * tiles;
-        const int j0 = t * TN;
-        const TIO *xb = x + (i64)b * C * L;
-#pragma unroll
-        for (int u = 0; u < TPT; ++u) {
-            const int task = tid + u * NT;
-            const int q = task % NQ, g = task / NQ;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) load_quad_raw<TIO, VL>(xb + (i64)(8 * g + i) * L, j0 + 4 * q, L, pf[u][i]);
-        }
-        const bool snap = tl == tile_end - 1 && t < tiles - 1;
-#pragma unroll
-        for (int u = 0; u < HPT; ++u) {
-            const int e = min(tid + u * NT, NH - 1);  // threads past NH repeat the last element
-            const int ic = e / (K - 1), kk = e - ic * (K - 1);
-            const int j = j0 + TN + kk;
-            ph[u] = *(snap ? halo + (i64)blockIdx.x * NH + e : xb + (i64)ic * L + (j < L ? j : L - 1));
-        }
-    };
-    // Converts the loaded tile tl (zero past L) into LDS buffer buf.
-    auto stage_tile = [&](int buf, int tl) __attribute__((always_inline)) {
-        __bf16 *hi = xs + buf * BUF;
-        const int b = tl / tiles, t = tl - b * tiles;
-        const int j0 = t * TN;
-        const bool snap = tl == tile_end - 1 && t < tiles - 1;
-#pragma unroll
-        for (int u = 0; u < TPT; ++u) {
-            const int task = tid + u * NT;
-            const int q = task % NQ, g = task / NQ;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const bool in = j0 + 4 * q + c < L;
-                bf16x8_t hv;
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    __bf16 hb;
-                    if constexpr (std::is_same<TIO, __hip_bfloat16>::value)
-                        hb = __builtin_bit_cast(__bf16, pf[u][i][c]);
-                    else
-                        hb = to_bf16(io_to_f32(pf[u][i][c]));
-                    hv[i] = in ? hb : (__bf16)0.0f;
-                }
-                *reinterpret_cast<bf16x8_t *>(hi + (4 * q + c) * RS + 8 * g) = hv;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < HPT; ++u) {
-            const int e = tid + u * NT;
-            const int ic = e / (K - 1), kk = e - ic * (K - 1);
-            const bool in = snap || j0 + TN + kk < L;
-            if (e < NH) hi[(TN + kk) * RS + ic] = in ? to_bf16(io_to_f32(ph[u])) : (__bf16)0.0f;
-        }
-    };
-
-    load_tile(tile);
-    stage_tile(0, tile);
-    __syncthreads();
-    const int jl = lane % 32, kh = 8 * (lane / 32);
-    float *red_out = red + ((ob * 2 + (1 - h)) * (NSUB / 2)) * 16 * 64;  // the partner's half
-    const float *red_in = red + ((ob * 2 + h) * (NSUB / 2)) * 16 * 64;   // this wave's half
-    for (int it = 0; tile < tile_end; ++it) {
-        const int next = tile + 1;
-        if (next < tile_end) load_tile(next);
-        const __bf16 *hi = xs + (it & 1) * BUF;
-        f32x16_t acc[NSUB];
-#pragma unroll
-        for (int n = 0; n < NSUB; ++n)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[n][r] = 0.0f;
-        // The k-loop is instantiated for each half (h is wave-uniform), so
-        // every LDS offset is a compile-time constant.
-        auto kloop = [&](auto hc_) __attribute__((always_inline)) {
-            constexpr int H = decltype(hc_)::value;
-            constexpr int NSTEP = KH * NSUB, PD = 2;
-            const __bf16 *base = hi + jl * RS + kh;
-            bf16x8_t bq[PD + 1];
-            auto bread = [&](auto ic_) __attribute__((always_inline)) {
-                constexpr int idx = decltype(ic_)::value;
-                constexpr int sl = idx / NSUB, n = idx % NSUB;
-                constexpr int sg = H * KH + sl;  // global k-step
-                constexpr int k = (16 * sg) / C, icb = 16 * sg - k * C;
-                bq[idx % (PD + 1)] = *reinterpret_cast<const bf16x8_t *>(base + (32 * n + k) * RS + icb);
-            };
-            static_for<0, PD>(bread);
-            static_for<0, NSTEP>([&](auto ic_) __attribute__((always_inline)) {
-                constexpr int idx = decltype(ic_)::value;
-                constexpr int sl = idx / NSUB, n = idx % NSUB;
-                if constexpr (idx + PD < NSTEP) bread(std::integral_constant<int, idx + PD>{});
-                acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[sl], bq[idx % (PD + 1)], acc[n], 0, 0, 0);
-            });
-        };
-        if (h == 0)
-            kloop(std::integral_constant<int, 0>{});
-        else
-            kloop(std::integral_constant<int, 1>{});
-        // Exchange partial sums: h = 0 keeps sub-tiles [0, NSUB/2), h = 1 the rest.
-#pragma unroll
-        for (int n2 = 0; n2 < NSUB / 2; ++n2) {
-            // the partner's sub-tile (n2 for h = 1, NSUB/2 + n2 for h = 0); selects keep acc in registers
-#pragma unroll
-            for (int r = 0; r < 16; ++r)
-                red_out[(n2 * 16 + r) * 64 + lane] = h ? acc[n2][r] : acc[NSUB / 2 + n2][r];
-        }
-        __syncthreads();
-        {
-            const int b = tile / tiles, t = tile - b * tiles;
-            TIO *xb = x + (i64)b * C * L;
-#pragma unroll
-            for (int n2 = 0; n2 < NSUB / 2; ++n2) {
-                const int n = h * (NSUB / 2) + n2;
-                const int j = t * TN + 32 * n + jl;
-                // D[m][n]: lane l, register r -> m = 8*(r/4) + 4*(l/32) + r%4, n = l%32.
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const float mine = h ? acc[NSUB / 2 + n2][r] : acc[n2][r];
-                    const float v = mine + red_in[(n2 * 16 + r) * 64 + lane];
-                    const int oc = 32 * ob + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
-                    if (j < Lout) xb[(i64)oc * L + j] = f32_to_io<TIO>(v + bias[oc]);
-                }
-            }
-        }
-        if (next < tile_end) stage_tile((it + 1) & 1, next);
-        __syncthreads();
-        tile = next;
-    }
-}
-
 // Plain 32x32x16 bf16 MFMA over one fragment pair, for the layout test:
 // a[l][e], b[l][e] are the lane fragments, d[l][r] the accumulator.
 __global__ void mfma_layout_probe_kernel(const __bf16 *a, const __bf16 *b, float *d) {
@@ -837,26 +658,6 @@ template <class TIO, int NPROD, int C, int K, int TN> void launch_mfma(void *x, 
                        (const float *)p.bias, (int)p.L, tiles, total, chunk);
 }
 
-template <class TIO, int C, int K, int TN> void launch_mfma_split(void *x, const ConvPlan &p, hipStream_t s) {
-    const int tiles = (int)ceil_div((u64)(p.L - p.K + 1), (u64)TN);
-    const int total = (int)(p.B * tiles);
-    int grid = 0, chunk = 0;
-    launch_halo<TIO>(x, p, C, K, TN, tiles, total, &grid, &chunk, s);
-    auto k = p.L % 4 == 0 ? conv_mfma_split_kernel<TIO, C, K, TN, true> : conv_mfma_split_kernel<TIO, C, K, TN, false>;
-    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds_bytes);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(4 * C), (size_t)p.lds_bytes, s, reinterpret_cast<TIO *>(x),
-                       reinterpret_cast<const TIO *>(p.halo), (const __bf16 *)p.w, (const float *)p.bias, (int)p.L,
-                       tiles, total, chunk);
-}
-
-template <class TIO, int C, int K, int TN> int occupancy_mfma_split(u64 lds_bytes) {
-    auto k = conv_mfma_split_kernel<TIO, C, K, TN, true>;
-    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 4 * C, (size_t)lds_bytes) != hipSuccess) return 0;
-    return blocks;
-}
-
 template <class TIO, int NPROD, int C, int K, int TN> int occupancy_mfma(u64 lds_bytes) {
     auto k = conv_mfma_kernel<TIO, NPROD, C, K, TN, true>;
     (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
@@ -870,26 +671,16 @@ struct MfmaVariant {
     int nprod, C, K, TN;
     MfmaLaunch fn;
     int (*occ)(u64 lds_bytes);
-    int split;  // conv_mfma_split_kernel (two waves per channel block)
 };
 
-#define MV(TIO, IO, NP, C, K, TN) \
-    {IO, NP, C, K, TN, &launch_mfma<TIO, NP, C, K, TN>, &occupancy_mfma<TIO, NP, C, K, TN>, 0}
-#define MS(TIO, IO, C, K, TN) \
-    {IO, 1, C, K, TN, &launch_mfma_split<TIO, C, K, TN>, &occupancy_mfma_split<TIO, C, K, TN>, 1}
-// First match wins.
+#define MV(TIO, IO, NP, C, K, TN) {IO, NP, C, K, TN, &launch_mfma<TIO, NP, C, K, TN>, &occupancy_mfma<TIO, NP, C, K, TN>}
 const MfmaVariant kMfmaVariants[] = {
-#ifndef BKMA_CONV_NO_SPLIT
-    MS(float, BKMA_F32, 128, 5, 128),              MS(__hip_bfloat16, BKMA_BF16, 128, 5, 128),
-    MS(float, BKMA_F32, 128, 3, 128),              MS(float, BKMA_F32, 64, 5, 128),
-#endif
     MV(float, BKMA_F32, 1, 128, 5, 128),           MV(float, BKMA_F32, 3, 128, 5, 64),
     MV(__hip_bfloat16, BKMA_BF16, 1, 128, 5, 128), MV(__hip_bfloat16, BKMA_BF16, 3, 128, 5, 64),
     MV(float, BKMA_F32, 1, 128, 3, 128),           MV(float, BKMA_F32, 3, 128, 3, 64),
     MV(float, BKMA_F32, 1, 64, 5, 128),            MV(float, BKMA_F32, 3, 64, 5, 64),
 };
 #undef MV
-#undef MS
 
 // Grid (one work-group per chunk of consecutive tiles) and the halo copies
 // of the chunks' last tiles.
@@ -1005,7 +796,6 @@ extern "C" bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype i
             return bail(fail(BKMA_ERR_DEVICE, "conv1d plan: bias copy"));
         const bool xlo = nprod == 3 && io_dtype != BKMA_BF16;
         p->lds_bytes = 2ull * (u64)(mv->TN + K - 1) * (u64)(C + 8) * 2 * (xlo ? 2 : 1);
-        if (mv->split) p->lds_bytes += (u64)(C / 32) * (u64)(mv->TN / 32) * 16 * 64 * 4;  // partial-sum exchange
         if (p->lds_bytes > lds_max) return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: LDS tile too large"));
         int dev = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
@@ -1015,7 +805,7 @@ extern "C" bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype i
         const u64 esz = io_dtype == BKMA_BF16 ? 2 : 4;
         if (hipMalloc(&p->halo, std::max<u64>((u64)p->resident_groups * C * (K - 1) * esz, 16)) != hipSuccess)
             return bail(fail(BKMA_ERR_OUT_OF_MEMORY, "conv1d plan: halo buffer"));
-        p->name = std::string(mv->split ? "conv_mfma_split_bf16" : "conv_mfma_bf16") + (nprod == 3 ? "x3" : "") + "<C" + std::to_string(C) + ",K" +
+        p->name = std::string("conv_mfma_bf16") + (nprod == 3 ? "x3" : "") + "<C" + std::to_string(C) + ",K" +
                   std::to_string(K) + ",TN" + std::to_string(mv->TN) + (io_dtype == BKMA_BF16 ? ",io_bf16" : ",io_f32") +
                   ">";
     } else if (mode == BKMA_CONV_FP8) {
