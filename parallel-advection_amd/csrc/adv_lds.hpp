@@ -60,6 +60,81 @@ __global__ void halo_snapshot_kernel(const T *__restrict__ data, T *__restrict__
     }
 }
 
+// Outputs of rows [r0, r0+nr) from the staged rows in LDS.  Thread (row
+// offset, chunk of V columns); the foot location and coefficients are
+// computed once per (row, chunk); chunks beyond the work-group loop.
+template <class T, bool SEG, int V>
+__device__ __forceinline__ void lds_compute(T *__restrict__ slab, const AdvDev &p, i64 i0, i64 r0, i64 nr, int hb,
+                                            i64 lds_rows, const T *__restrict__ lds) {
+    using VT = T __attribute__((ext_vector_type(V)));
+    const i64 n1 = p.n1, n2 = p.n2;
+    const int nthr = blockDim.x;
+    const i64 nch = n2 / V;
+    const int cw = (int)min<i64>(nch, nthr);
+    const int rows_per_pass = nthr / cw;
+    const int t_row = threadIdx.x / cw, t_col = threadIdx.x - t_row * cw;
+    const int half = (int)(n1 / 2);
+    double coef[6];
+    unsigned long long cached = ~0ull;
+    for (i64 rr = t_row; t_row < rows_per_pass && rr < nr; rr += rows_per_pass) {
+        const i64 i1 = r0 + rr;
+        double d_prev1;
+        const int leftNode = adv_foot(p, (int)i0, (int)i1, d_prev1);
+        const unsigned long long bits = __builtin_bit_cast(unsigned long long, d_prev1);
+        if (bits != cached) {
+            adv_lag_basis(d_prev1, coef);
+            cached = bits;
+        }
+        const int ipos1 = leftNode - 2;
+        i64 lrow;  // LDS row of the first stencil point
+        bool ok = true;
+        if constexpr (SEG) {
+            int delta = ipos1 - (int)i1;
+            if (delta < -half) delta += (int)n1;
+            if (delta > half) delta -= (int)n1;
+            lrow = hb + rr + delta;
+            ok = lrow >= 0 && lrow + 5 < lds_rows;
+        } else {
+            lrow = stencil_fast_ok(n1, ipos1) ? stencil_base(n1, ipos1) : stencil_index_slow(n1, ipos1, 0);
+        }
+        for (i64 ch = t_col; ch < nch; ch += cw) {
+            double acc[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) acc[v] = 0.;
+            i64 r = lrow;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                T f[V];
+                if constexpr (V == 1) {
+                    f[0] = lds[r * n2 + ch];
+                } else {
+                    const VT fv = *reinterpret_cast<const VT *>(lds + r * n2 + ch * V);
+#pragma unroll
+                    for (int v = 0; v < V; ++v) f[v] = fv[v];
+                }
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const double fd = ok ? (double)f[v] : (double)NAN;
+                    if constexpr (sizeof(T) == 8) acc[v] = acc[v] + coef[k] * fd;
+                    else acc[v] = fma(coef[k], fd, acc[v]);
+                }
+                r += 1;
+                if constexpr (!SEG) {
+                    if (r >= n1) r -= n1;
+                }
+            }
+            if constexpr (V == 1) {
+                slab[i1 * n2 + ch] = (T)acc[0];
+            } else {
+                VT o;
+#pragma unroll
+                for (int v = 0; v < V; ++v) o[v] = (T)acc[v];
+                *reinterpret_cast<VT *>(slab + i1 * n2 + ch * V) = o;
+            }
+        }
+    }
+}
+
 // One LDS-staged unit: rows [r0, r1) of slab i0 (all n2 columns).  Every
 // thread of the work-group calls it; it contains one barrier and no early
 // return, so it can run inside loops of a persistent kernel.
@@ -96,50 +171,12 @@ __device__ __forceinline__ void line_lds_unit(T *__restrict__ slab, const T *__r
     }
     __syncthreads();
 
-    // Compute: thread (row offset, column); columns beyond nthr loop.
-    const int cw = (int)min<i64>(n2, nthr);
-    const int rows_per_pass = nthr / cw;
-    const int t_row = threadIdx.x / cw, t_col = threadIdx.x - t_row * cw;
-    const int half = (int)(n1 / 2);
-    double coef[6];
-    unsigned long long cached = ~0ull;
-    for (i64 rr = t_row; t_row < rows_per_pass && rr < nr; rr += rows_per_pass) {
-        const i64 i1 = r0 + rr;
-        double d_prev1;
-        const int leftNode = adv_foot(p, (int)i0, (int)i1, d_prev1);
-        const unsigned long long bits = __builtin_bit_cast(unsigned long long, d_prev1);
-        if (bits != cached) {
-            adv_lag_basis(d_prev1, coef);
-            cached = bits;
-        }
-        const int ipos1 = leftNode - 2;
-        i64 lrow;  // LDS row of the first stencil point
-        bool ok = true;
-        if constexpr (SEG) {
-            int delta = ipos1 - (int)i1;
-            if (delta < -half) delta += (int)n1;
-            if (delta > half) delta -= (int)n1;
-            lrow = hb + rr + delta;
-            ok = lrow >= 0 && lrow + 5 < lds_rows;
-        } else {
-            lrow = stencil_fast_ok(n1, ipos1) ? stencil_base(n1, ipos1) : stencil_index_slow(n1, ipos1, 0);
-        }
-        for (i64 c = t_col; c < n2; c += cw) {
-            double acc = 0.;
-            i64 r = lrow;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                const double f = ok ? (double)lds[r * n2 + c] : (double)NAN;
-                if constexpr (sizeof(T) == 8) acc = acc + coef[k] * f;
-                else acc = fma(coef[k], f, acc);
-                r += 1;
-                if constexpr (!SEG) {
-                    if (r >= n1) r -= n1;
-                }
-            }
-            slab[i1 * n2 + c] = (T)acc;
-        }
-    }
+    // Compute: 16-byte column chunks when n2 allows (one foot location per
+    // chunk, vector LDS reads and global stores), else single columns.
+    if (n2 % (16 / (i64)sizeof(T)) == 0)
+        lds_compute<T, SEG, 16 / (int)sizeof(T)>(slab, p, i0, r0, nr, hb, lds_rows, lds);
+    else
+        lds_compute<T, SEG, 1>(slab, p, i0, r0, nr, hb, lds_rows, lds);
 }
 
 template <class T, bool SEG>

@@ -256,10 +256,12 @@ extern "C" bkma_status bkma_adv_batch_create(const bkma_adv_problem *problems, i
     for (int32_t j = 0; j < nproblems; ++j) {
         const bkma_adv_params &p = problems[j].params;
         const double smax = std::fabs(p.dt * p.inv_dx) * std::max(std::fabs(p.minRealVx), std::fabs(p.maxRealVx));
-        if (!problems[j].data || p.n0 == 0 || p.n1 == 0 || p.n2 == 0 || p.n0 > 0x7fffffffULL ||
+        if (!problems[j].data || reinterpret_cast<uintptr_t>(problems[j].data) % 16 != 0 || p.n0 == 0 || p.n1 == 0 ||
+            p.n2 == 0 || p.n0 > 0x7fffffffULL ||
             p.n1 > 0x7fffffffULL || !std::isfinite(p.dt) || !(p.realWidthX > 0) || !(smax < (double)p.n1 / 2 - 8)) {
             delete b;
-            return fail(BKMA_ERR_INVALID_ARGUMENT, "advection batch: problem " + std::to_string(j) + " is invalid");
+            return fail(BKMA_ERR_INVALID_ARGUMENT,
+                        "advection batch: problem " + std::to_string(j) + " is invalid (or its data is not 16-byte aligned)");
         }
         b->problems.push_back(problems[j]);
         b->cells += p.n0 * p.n1 * p.n2;
