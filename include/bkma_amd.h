@@ -197,7 +197,8 @@ typedef struct bkma_tuned_config {
     int32_t seg_rows;     /* LINE_LDS: rows per segment (0 = whole line in LDS) */
     int32_t wide_window;  /* COLUMN_STREAM, F32 storage: 1 = FP64 register window (rows widened once) */
     int32_t steps_per_pass; /* COLUMN_STREAM: time steps fused into one pass over HBM (0 or 1: one) */
-    int32_t _reserved[2];
+    int32_t chunk_segs;   /* LINE_LDS segmented: consecutive segments one work-group walks in order (0 or 1: one) */
+    int32_t _reserved;
 } bkma_tuned_config;
 
 typedef struct bkma_adv_plan bkma_adv_plan;

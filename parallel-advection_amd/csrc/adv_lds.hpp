@@ -20,10 +20,12 @@ namespace bkma {
 
 struct LdsArgs {
     AdvDev p;
-    i64 seg_rows;  // rows per segment (== n1 in whole-line mode)
-    i64 nseg;      // segments per slab
-    i64 hb_max;    // halo rows before a segment (allocation bound)
-    i64 ha_max;    // halo rows after a segment (allocation bound)
+    i64 seg_rows;    // rows per segment (== n1 in whole-line mode)
+    i64 nseg;        // segments per slab
+    i64 hb_max;      // halo rows before a segment (allocation bound)
+    i64 ha_max;      // halo rows after a segment (allocation bound)
+    i64 chunk_segs;  // segmented mode: consecutive segments one work-group walks in order
+    i64 nchunk;      // chunks per slab
 };
 
 // Halo extents of slab i0: rows [r0 - hb, r0) and [r1, r1 + ha).
@@ -189,6 +191,74 @@ __global__ __launch_bounds__(256) void adv_line_lds_kernel(T *__restrict__ data,
     const i64 r1 = min(a.p.n1, r0 + a.seg_rows);
     line_lds_unit<T, SEG>(data + i0 * a.p.n1 * a.p.n2, SEG ? halo + unit * (a.hb_max + a.ha_max) * a.p.n2 : nullptr,
                           a.p, i0, r0, r1, a.hb_max, a.ha_max, reinterpret_cast<T *>(lds_raw));
+}
+
+// Rows [c0, c1) of slab i0 ("chunk"), walked by one work-group in segments
+// of seg_rows in increasing order.  Only the chunk's own rows are read from
+// HBM besides the halo snapshot of the chunk (hsnap: hb rows before c0, then
+// ha rows after c1): the halo rows before a later segment are old rows of the
+// previous segment, still in LDS, moved to the front (seg_rows >= hb_max);
+// the rows after a segment are read from HBM while they are still unwritten.
+// Every thread calls it; the loop is uniform over the work-group.
+template <class T>
+__device__ __forceinline__ void line_chunk_unit(T *__restrict__ slab, const T *__restrict__ hsnap, const AdvDev &p,
+                                                i64 i0, i64 c0, i64 c1, i64 seg_rows, i64 hb_max, i64 ha_max,
+                                                T *__restrict__ lds) {
+    const i64 n2 = p.n2;
+    const int nthr = blockDim.x;
+    int hb, ha;
+    slab_halo(p, i0, hb_max, ha_max, hb, ha);
+    for (i64 r0 = c0; r0 < c1; r0 += seg_rows) {
+        const i64 r1 = min(c1, r0 + seg_rows);
+        const i64 nr = r1 - r0;
+        const i64 lds_rows = hb + nr + ha;
+        if (r0 == c0)
+            for (i64 e = threadIdx.x; e < (i64)hb * n2; e += nthr) lds[e] = hsnap[e];
+        {
+            const i64 cnt = nr * n2;
+            const T *src = slab + r0 * n2;
+            T *dst = lds + (i64)hb * n2;
+            constexpr int V = 16 / sizeof(T);
+            const bool vec_ok = ((reinterpret_cast<uintptr_t>(src) | (uintptr_t)(hb * n2 * sizeof(T))) % 16) == 0;
+            if (vec_ok) {
+                using VT = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+                const i64 nv = cnt / V;
+                for (i64 e = threadIdx.x; e < nv; e += nthr)
+                    reinterpret_cast<VT *>(dst)[e] = reinterpret_cast<const VT *>(src)[e];
+                for (i64 e = nv * V + threadIdx.x; e < cnt; e += nthr) dst[e] = src[e];
+            } else {
+                for (i64 e = threadIdx.x; e < cnt; e += nthr) dst[e] = src[e];
+            }
+            T *adst = lds + (hb + nr) * n2;
+            for (i64 e = threadIdx.x; e < (i64)ha * n2; e += nthr) {
+                const i64 g = r1 + e / n2;  // global row
+                adst[e] = g < c1 ? slab[r1 * n2 + e] : hsnap[(i64)hb * n2 + (g - c1) * n2 + e % n2];
+            }
+        }
+        __syncthreads();
+        if (n2 % (16 / (i64)sizeof(T)) == 0)
+            lds_compute<T, true, 16 / (int)sizeof(T)>(slab, p, i0, r0, nr, hb, lds_rows, lds);
+        else
+            lds_compute<T, true, 1>(slab, p, i0, r0, nr, hb, lds_rows, lds);
+        __syncthreads();  // every stencil read of this segment is done
+        if (r1 < c1) {
+            for (i64 e = threadIdx.x; e < (i64)hb * n2; e += nthr) lds[e] = lds[nr * n2 + e];
+            __syncthreads();
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void adv_line_chunk_kernel(T *__restrict__ data, const T *__restrict__ halo,
+                                                             LdsArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const i64 unit = blockIdx.x;
+    const i64 i0 = unit / a.nchunk, ch = unit - i0 * a.nchunk;
+    const i64 chunk_rows = a.seg_rows * a.chunk_segs;
+    const i64 c0 = ch * chunk_rows;
+    const i64 c1 = min(a.p.n1, c0 + chunk_rows);
+    line_chunk_unit<T>(data + i0 * a.p.n1 * a.p.n2, halo + unit * (a.hb_max + a.ha_max) * a.p.n2, a.p, i0, c0, c1,
+                       a.seg_rows, a.hb_max, a.ha_max, reinterpret_cast<T *>(lds_raw));
 }
 
 }  // namespace bkma

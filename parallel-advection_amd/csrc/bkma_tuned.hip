@@ -135,7 +135,7 @@ bkma_status stream_feasible(const bkma_adv_params &p, bkma_dtype dt, const Strea
 }
 
 // LINE_LDS shape for a given segment length (0 = whole line).
-bkma_status lds_shape(const bkma_adv_params &p, bkma_dtype dt, i64 seg_rows, LdsArgs *a, u64 *bytes,
+bkma_status lds_shape(const bkma_adv_params &p, bkma_dtype dt, i64 seg_rows, i64 chunk_segs, LdsArgs *a, u64 *bytes,
                       std::string *why) {
     if (!shift_supported(p, why)) return BKMA_ERR_UNSUPPORTED;
     u64 lds_max = 0;
@@ -154,6 +154,12 @@ bkma_status lds_shape(const bkma_adv_params &p, bkma_dtype dt, i64 seg_rows, Lds
         a->hb_max = std::max(0, dhi + 2);
         a->ha_max = std::max(0, 3 - dlo);
     }
+    a->chunk_segs = a->nseg > 1 ? std::max<i64>(1, std::min<i64>(chunk_segs, a->nseg)) : 1;
+    a->nchunk = (i64)ceil_div((u64)a->nseg, (u64)a->chunk_segs);
+    if (a->chunk_segs > 1 && a->seg_rows < a->hb_max) {
+        if (why) *why = "segments shorter than the halo cannot be chained";
+        return BKMA_ERR_UNSUPPORTED;
+    }
     *bytes = (u64)(a->seg_rows + a->hb_max + a->ha_max) * p.n2 * esz;
     if (*bytes > lds_max) {
         if (why) *why = "segment does not fit in LDS";
@@ -171,11 +177,16 @@ bkma_status launch_lds(const TunedPlan &pl, void *data, hipStream_t s) {
     const LdsArgs &a = pl.lds;
     const unsigned units = (unsigned)((i64)pl.params.n0 * a.nseg);
     if (a.nseg > 1) {
-        hipLaunchKernelGGL(halo_snapshot_kernel<T>, dim3(units), dim3(256), 0, s, (const T *)data, (T *)pl.halo, a);
+        // halo snapshot per chunk: the chunk's rows seen as one long segment
+        LdsArgs sa = a;
+        sa.seg_rows = a.seg_rows * a.chunk_segs;
+        sa.nseg = a.nchunk;
+        const unsigned cunits = (unsigned)((i64)pl.params.n0 * a.nchunk);
+        hipLaunchKernelGGL(halo_snapshot_kernel<T>, dim3(cunits), dim3(256), 0, s, (const T *)data, (T *)pl.halo, sa);
         BKMA_TRY(check_launch("halo_snapshot_kernel"));
-        auto k = adv_line_lds_kernel<T, true>;
+        auto k = adv_line_chunk_kernel<T>;
         BKMA_HIP_TRY(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds_bytes));
-        hipLaunchKernelGGL(k, dim3(units), dim3(256), (size_t)pl.lds_bytes, s, (T *)data, (const T *)pl.halo, a);
+        hipLaunchKernelGGL(k, dim3(cunits), dim3(256), (size_t)pl.lds_bytes, s, (T *)data, (const T *)pl.halo, a);
     } else {
         auto k = adv_line_lds_kernel<T, false>;
         BKMA_HIP_TRY(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds_bytes));
@@ -220,7 +231,10 @@ std::string stream_name(const StreamVariant &v, int ns, bool persistent) {
 
 std::string lds_name(bkma_dtype dt, const LdsArgs &a) {
     return std::string("line_lds<") + (dt == BKMA_F64 ? "f64" : "f32") +
-           (a.nseg > 1 ? ",seg" + std::to_string(a.seg_rows) + "x" + std::to_string(a.nseg) : ",whole_line") + ">";
+           (a.nseg > 1 ? ",seg" + std::to_string(a.seg_rows) + "x" + std::to_string(a.nseg) + ",chunk" +
+                             std::to_string(a.chunk_segs)
+                       : ",whole_line") +
+           ">";
 }
 
 // Resident work-groups on the whole device for a variant (persistent grid).
@@ -263,17 +277,18 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
         if (steps_per_pass(cfg) != 1)
             return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: LINE_LDS runs one step per pass");
         pl->cfg.steps_per_pass = 1;
-        if (lds_shape(p, dt, cfg.seg_rows, &pl->lds, &pl->lds_bytes, &why) != BKMA_OK)
+        if (lds_shape(p, dt, cfg.seg_rows, cfg.chunk_segs, &pl->lds, &pl->lds_bytes, &why) != BKMA_OK)
             return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
         if (pl->lds.nseg > 1) {
             const u64 esz = dt == BKMA_F64 ? 8 : 4;
-            const u64 bytes = p.n0 * (u64)pl->lds.nseg * (u64)(pl->lds.hb_max + pl->lds.ha_max) * p.n2 * esz;
+            const u64 bytes = p.n0 * (u64)pl->lds.nchunk * (u64)(pl->lds.hb_max + pl->lds.ha_max) * p.n2 * esz;
             if (hipMalloc(&pl->halo, std::max<u64>(bytes, 16)) != hipSuccess) {
                 pl->halo = nullptr;
                 return fail(BKMA_ERR_OUT_OF_MEMORY, "tuned advection: halo buffer allocation failed");
             }
         }
         pl->cfg.seg_rows = pl->lds.nseg > 1 ? (int32_t)pl->lds.seg_rows : 0;
+        pl->cfg.chunk_segs = (int32_t)pl->lds.chunk_segs;
         pl->name = lds_name(dt, pl->lds);
         return BKMA_OK;
     }
@@ -319,15 +334,19 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
         if (rows >= 32 && (u64)rows < p.n1) segs.push_back(rows & ~(i64)3);
     }
     for (i64 sr : segs) {
-        if (lds_shape(p, dt, sr, &a, &bytes, why) != BKMA_OK) continue;
-        bkma_tuned_config c;
-        std::memset(&c, 0, sizeof(c));
-        c.strategy = BKMA_STRATEGY_LINE_LDS;
-        c.steps_per_pass = 1;
-        c.seg_rows = (int32_t)(a.nseg > 1 ? a.seg_rows : 0);
-        bool dup = false;
-        for (auto &o : out) dup |= o.strategy == c.strategy && o.seg_rows == c.seg_rows;
-        if (!dup) out.push_back(c);
+        for (i64 cs : {1, 4, 16, 64}) {
+            if (lds_shape(p, dt, sr, cs, &a, &bytes, why) != BKMA_OK) continue;
+            bkma_tuned_config c;
+            std::memset(&c, 0, sizeof(c));
+            c.strategy = BKMA_STRATEGY_LINE_LDS;
+            c.steps_per_pass = 1;
+            c.seg_rows = (int32_t)(a.nseg > 1 ? a.seg_rows : 0);
+            c.chunk_segs = (int32_t)a.chunk_segs;
+            bool dup = false;
+            for (auto &o : out)
+                dup |= o.strategy == c.strategy && o.seg_rows == c.seg_rows && o.chunk_segs == c.chunk_segs;
+            if (!dup) out.push_back(c);
+        }
     }
     return out;
 }
@@ -344,7 +363,8 @@ int heuristic_score(const bkma_tuned_config &c, bkma_dtype dt) {
         return 1000000000 - c.smax * 1000000 - std::abs(ns - want) * 100000 + c.vec * 1000 + c.prefetch * 10 +
                c.nontemporal - c.persistent - 100 * c.wide_window;
     }
-    return c.seg_rows == 0 ? 500000000 : 100000000 + std::min(c.seg_rows, 1 << 24) / 1024;
+    return c.seg_rows == 0 ? 500000000
+                           : 100000000 + std::min(c.seg_rows, 1 << 24) / 1024 - std::abs(c.chunk_segs - 16) * 10;
 }
 
 bkma_status time_plan(const TunedPlan &pl, void *buf, int reps, hipStream_t s, float *ms) {

@@ -158,31 +158,38 @@ def test_unsupported_shapes_are_reported(cuda):
 # ---------------------------------------------------------------------------
 # LINE_LDS: slabs (or segments of them, with halo snapshots) staged in LDS
 # ---------------------------------------------------------------------------
-def _lds_cfg(seg_rows):
-    return B.TunedConfig(strategy=B.STRATEGY_LINE_LDS, seg_rows=seg_rows)
+def _lds_cfg(seg_rows, chunk_segs=1):
+    return B.TunedConfig(strategy=B.STRATEGY_LINE_LDS, seg_rows=seg_rows, chunk_segs=chunk_segs)
 
 
 LDS_CASES = [
-    # (n0, n1, n2, dt, seg_rows)
-    (8, 1024, 1, 1e-3, 0),        # whole line, the FP64 parity configuration's shape
-    (5, 777, 3, 2e-3, 0),         # odd sizes
-    (3, 2048, 8, 4e-3, 0),        # shift ~8 cells
-    (3, 4096, 8, 2e-3, 1000),     # same, segmented (ragged last segment)
-    (2, 16384, 4, 1e-3, 1024),    # long line, shift ~16 cells, 16 segments
-    (4, 2048, 1, 1e-2, 256),      # shift ~20 cells, segments shorter than the halo span
-    (3, 256, 64, 2e-2, 0),        # n2 = 64 with a shift (~5 cells) beyond the register window
+    # (n0, n1, n2, dt, seg_rows, chunk_segs)
+    (8, 1024, 1, 1e-3, 0, 1),        # whole line, the FP64 parity configuration's shape
+    (5, 777, 3, 2e-3, 0, 1),         # odd sizes
+    (3, 2048, 8, 4e-3, 0, 1),        # shift ~8 cells
+    (3, 4096, 8, 2e-3, 1000, 1),     # same, segmented (ragged last segment)
+    (3, 4096, 8, 2e-3, 1000, 2),     # chained segments, ragged last chunk
+    (2, 16384, 4, 1e-3, 1024, 1),    # long line, shift ~16 cells, 16 segments
+    (2, 16384, 4, 1e-3, 1024, 4),    # 4 chunks of 4 segments
+    (2, 16384, 4, 1e-3, 1024, 16),   # one chunk per slab (halo only at the wrap)
+    (2, 16384, 4, 1e-3, 1024, 3),    # chunks not dividing the segments
+    (4, 2048, 1, 1e-2, 256, 1),      # shift ~20 cells, segments shorter than the halo span
+    (4, 2048, 1, 1e-2, 256, 5),      # same, chained (n2 = 1: scalar path)
+    (3, 2048, 3, 3e-2, 128, 4),      # shift ~61 cells, odd n2
+    (3, 256, 64, 2e-2, 0, 1),        # n2 = 64 with a shift (~5 cells) beyond the register window
 ]
 
 
-@pytest.mark.parametrize("case", LDS_CASES, ids=lambda c: "x".join(map(str, c[:3])) + f"-dt{c[3]}-seg{c[4]}")
+@pytest.mark.parametrize("case", LDS_CASES,
+                         ids=lambda c: "x".join(map(str, c[:3])) + f"-dt{c[3]}-seg{c[4]}-chunk{c[5]}")
 @pytest.mark.parametrize("dtype", [B.F64, B.F32], ids=["f64", "f32"])
 def test_line_lds_bit_exact(cuda, case, dtype):
-    n0, n1, n2, dt, seg = case
+    n0, n1, n2, dt, seg, chunk = case
     p = B.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
     po = O.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
     npdt = np.float64 if dtype == B.F64 else np.float32
     host = O.uniform(n0 * n1 * n2, 7, npdt).reshape(n0, n1, n2)
-    got, spp = _run(host, p, dtype, 4, config=_lds_cfg(seg))
+    got, spp = _run(host, p, dtype, 4, config=_lds_cfg(seg, chunk))
     assert spp == 1
     ref = host.copy()
     if dtype == B.F64:
@@ -190,6 +197,13 @@ def test_line_lds_bit_exact(cuda, case, dtype):
     else:
         O.advect_f32_mixed(ref, po, 4)
     assert np.array_equal(got, ref), np.abs(got.astype(np.float64) - ref).max()
+
+
+def test_chained_segments_shorter_than_halo_rejected(cuda):
+    # shift ~61 cells: 32-row segments cannot carry their halo to the next one
+    p = B.make_params(2, 2048, 4, dt=3e-2, minRealVx=-1, maxRealVx=1)
+    with pytest.raises(B.BkmaUnsupported):
+        B.AdvPlan(p, B.F32, config=_lds_cfg(32, 4))
 
 
 def test_autotune_small_n2_long_line(cuda):
