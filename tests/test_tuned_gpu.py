@@ -200,8 +200,9 @@ def test_line_lds_bit_exact(cuda, case, dtype):
 
 
 def test_chained_segments_shorter_than_halo_rejected(cuda):
-    # shift ~61 cells: 32-row segments cannot carry their halo to the next one
-    p = B.make_params(2, 2048, 4, dt=3e-2, minRealVx=-1, maxRealVx=1)
+    # v in {-1, -0.5, 0, 0.5}: backward shifts up to ~31 cells need 33 halo
+    # rows before a segment, more than a 32-row segment can carry to the next
+    p = B.make_params(4, 2048, 4, dt=3e-2, minRealVx=-1, maxRealVx=1)
     with pytest.raises(B.BkmaUnsupported):
         B.AdvPlan(p, B.F32, config=_lds_cfg(32, 4))
 
