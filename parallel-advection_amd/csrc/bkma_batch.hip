@@ -1,10 +1,12 @@
 // Ragged batches of independent advection problems in one persistent launch.
 //
 // Each problem ([n0][n1][n2], its own dt and velocity grid) is cut into work
-// items of the LINE_LDS kind: (slab i0, segment).  A problem whose slab fits
-// the LDS budget is one item per slab (whole line, periodic wrap in LDS);
-// a longer one is cut into segments whose halo rows come from a snapshot
-// buffer filled by batch_halo_kernel just before.  One step of the batch is
+// items of the LINE_LDS kind.  A problem whose slab fits the LDS budget is
+// one item per slab (whole line, periodic wrap in LDS); a longer one is cut
+// into segments, and an item is a chunk of consecutive segments of one slab
+// that a work-group walks in order (line_chunk_unit): only the halo rows
+// around the chunk come from a snapshot buffer filled by batch_halo_kernel
+// just before.  One step of the batch is
 //   batch_halo_kernel  (only if some problem is segmented)
 //   adv_batch_kernel   (persistent: work-groups pull items from a counter)
 // and `nsteps` steps are captured once into a hipGraph and replayed.
@@ -27,14 +29,15 @@ namespace bkma {
 
 struct BatchProblemDev {
     void *data;
-    void *halo;  // segmented problems: snapshot buffer (nullptr otherwise)
+    void *halo;  // segmented problems: snapshot buffer per (slab, chunk) (nullptr otherwise)
     AdvDev p;
     i64 seg_rows, nseg, hb_max, ha_max;
+    i64 chunk_segs, nchunk;  // segmented problems: segments walked in order by one item
 };
 
 struct BatchItem {
     int32_t prob;
-    int32_t seg;
+    int32_t chunk;  // segmented problems: chunk of chunk_segs segments (else 0)
     i64 i0;
 };
 
@@ -54,33 +57,35 @@ __global__ __launch_bounds__(256) void adv_batch_kernel(const BatchProblemDev *_
         const BatchItem item = items[it];
         const BatchProblemDev &pr = probs[item.prob];
         const i64 n1 = pr.p.n1, n2 = pr.p.n2;
-        const i64 r0 = (i64)item.seg * pr.seg_rows;
-        const i64 r1 = min(n1, r0 + pr.seg_rows);
         T *slab = reinterpret_cast<T *>(pr.data) + item.i0 * n1 * n2;
         if (pr.nseg > 1) {
-            const T *hsrc = reinterpret_cast<const T *>(pr.halo) +
-                            (item.i0 * pr.nseg + item.seg) * (pr.hb_max + pr.ha_max) * n2;
-            line_lds_unit<T, true>(slab, hsrc, pr.p, item.i0, r0, r1, pr.hb_max, pr.ha_max,
-                                   reinterpret_cast<T *>(lds_raw));
+            const i64 chunk_rows = pr.seg_rows * pr.chunk_segs;
+            const i64 c0 = (i64)item.chunk * chunk_rows;
+            const i64 c1 = min(n1, c0 + chunk_rows);
+            const T *hsnap = reinterpret_cast<const T *>(pr.halo) +
+                             (item.i0 * pr.nchunk + item.chunk) * (pr.hb_max + pr.ha_max) * n2;
+            line_chunk_unit<T>(slab, hsnap, pr.p, item.i0, c0, c1, pr.seg_rows, pr.hb_max, pr.ha_max,
+                               reinterpret_cast<T *>(lds_raw));
         } else {
-            line_lds_unit<T, false>(slab, nullptr, pr.p, item.i0, r0, r1, 0, 0, reinterpret_cast<T *>(lds_raw));
+            line_lds_unit<T, false>(slab, nullptr, pr.p, item.i0, 0, n1, 0, 0, reinterpret_cast<T *>(lds_raw));
         }
         __syncthreads();  // LDS is reused by the next item
     }
 }
 
-// Halo snapshot of the segmented items (one work-group per item).
+// Halo snapshot of the segmented items (one work-group per item): the hb
+// rows before and the ha rows after the item's chunk.
 template <class T>
 __global__ void batch_halo_kernel(const BatchProblemDev *__restrict__ probs, const BatchItem *__restrict__ items) {
     const BatchItem item = items[blockIdx.x];
     const BatchProblemDev &pr = probs[item.prob];
     const i64 n1 = pr.p.n1, n2 = pr.p.n2;
-    const i64 r0 = (i64)item.seg * pr.seg_rows;
-    const i64 r1 = min(n1, r0 + pr.seg_rows);
+    const i64 r0 = (i64)item.chunk * pr.seg_rows * pr.chunk_segs;
+    const i64 r1 = min(n1, r0 + pr.seg_rows * pr.chunk_segs);
     int hb, ha;
     slab_halo(pr.p, item.i0, pr.hb_max, pr.ha_max, hb, ha);
     const T *slab = reinterpret_cast<const T *>(pr.data) + item.i0 * n1 * n2;
-    T *dst = reinterpret_cast<T *>(pr.halo) + (item.i0 * pr.nseg + item.seg) * (pr.hb_max + pr.ha_max) * n2;
+    T *dst = reinterpret_cast<T *>(pr.halo) + (item.i0 * pr.nchunk + item.chunk) * (pr.hb_max + pr.ha_max) * n2;
     const i64 total = (i64)(hb + ha) * n2;
     for (i64 e = threadIdx.x; e < total; e += blockDim.x) {
         const i64 hr = e / n2, c = e - hr * n2;
@@ -95,6 +100,8 @@ __global__ void batch_halo_kernel(const BatchProblemDev *__restrict__ probs, con
 // Host side
 // ---------------------------------------------------------------------------
 namespace {
+
+constexpr i64 kBatchChunkSegs = 16;  // segments one item walks in order
 
 void shift_bounds(const bkma_adv_params &p, int *dlo, int *dhi) {
     const double v0 = p.minRealVx, v1 = p.minRealVx + (double)(p.n0 - 1) * p.dvx;
@@ -171,6 +178,7 @@ static bkma_status build_items(AdvBatch &b, u64 budget) {
         if (slab_bytes <= budget) {
             d.seg_rows = (i64)p.n1;
             d.nseg = 1;
+            d.chunk_segs = d.nchunk = 1;
         } else {
             int dlo, dhi;
             shift_bounds(p, &dlo, &dhi);
@@ -182,16 +190,20 @@ static bkma_status build_items(AdvBatch &b, u64 budget) {
                                                       " has rows too wide or a shift too large for the LDS budget");
             d.seg_rows = rows & ~(i64)3;
             d.nseg = (i64)ceil_div(p.n1, (u64)d.seg_rows);
+            // chain up to kBatchChunkSegs segments per item (needs segments at
+            // least as long as the halo before them)
+            d.chunk_segs = d.seg_rows >= d.hb_max ? std::min<i64>(d.nseg, kBatchChunkSegs) : 1;
+            d.nchunk = (i64)ceil_div((u64)d.nseg, (u64)d.chunk_segs);
             void *h = nullptr;
-            const u64 hb = p.n0 * (u64)d.nseg * (u64)(d.hb_max + d.ha_max) * p.n2 * esz;
+            const u64 hb = p.n0 * (u64)d.nchunk * (u64)(d.hb_max + d.ha_max) * p.n2 * esz;
             BKMA_HIP_TRY(hipMalloc(&h, std::max<u64>(hb, 16)));
             b.halos.push_back(h);
             d.halo = h;
         }
         b.host_probs.push_back(d);
         for (i64 i0 = 0; i0 < (i64)p.n0; ++i0)
-            for (i64 sg = 0; sg < d.nseg; ++sg) {
-                BatchItem it{(int32_t)j, (int32_t)sg, i0};
+            for (i64 ch = 0; ch < d.nchunk; ++ch) {
+                BatchItem it{(int32_t)j, (int32_t)ch, i0};
                 b.host_items.push_back(it);
                 if (d.nseg > 1) b.host_seg_items.push_back(it);
             }
@@ -200,7 +212,7 @@ static bkma_status build_items(AdvBatch &b, u64 budget) {
     std::stable_sort(b.host_items.begin(), b.host_items.end(), [&](const BatchItem &x, const BatchItem &y) {
         auto cost = [&](const BatchItem &i) {
             const auto &d = b.host_probs[i.prob];
-            return (double)std::min<i64>(d.seg_rows, d.p.n1) * (double)d.p.n2;
+            return (double)std::min<i64>(d.seg_rows * d.chunk_segs, d.p.n1) * (double)d.p.n2;
         };
         return cost(x) > cost(y);
     });

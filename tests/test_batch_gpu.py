@@ -76,8 +76,11 @@ def test_batch_graph_replays_bit_exact(cuda, dtype, autotune):
 
 def test_batch_without_graph_and_segments(cuda):
     import torch
-    # long lines force segments at the 32 KB budget of the heuristic-free path
-    specs = [(3, 8192, 8, 2e-3), (2, 16384, 1, 1e-3), (4, 640, 64, 1e-3), (5, 100, 1, 2e-3)]
+    # long lines force segments at the 32 KB budget of the heuristic-free path;
+    # the last problem's shift (~410 cells) exceeds its segment length, so its
+    # segments are not chained (one segment per item)
+    specs = [(3, 8192, 8, 2e-3), (2, 16384, 1, 1e-3), (4, 640, 64, 1e-3), (5, 100, 1, 2e-3),
+             (2, 16384, 8, 2.5e-2)]
     arrays, params = _make(B.F32, specs, 200)
     batch = B.AdvBatch(arrays, params, B.F32, autotune=False)
     info = batch.describe()
