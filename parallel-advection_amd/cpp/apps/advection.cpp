@@ -52,10 +52,14 @@ int main(int argc, char **argv) {
         }
 
         const auto start = std::chrono::high_resolution_clock::now();
-        for (size_t t = 0; t < maxIter; ++t) {
-            if (tuned) plan->run(Q, data, 1);
-            else bkma_run_function(Q, data, solver, optim_params, scratch);
+        if (tuned) {
+            plan->run(Q, data, maxIter);  // the plan may fuse several steps per pass over HBM
             Q.wait();
+        } else {
+            for (size_t t = 0; t < maxIter; ++t) {
+                bkma_run_function(Q, data, solver, optim_params, scratch);
+                Q.wait();
+            }
         }
         const auto end = std::chrono::high_resolution_clock::now();
         const std::chrono::duration<double> elapsed_seconds = end - start;
