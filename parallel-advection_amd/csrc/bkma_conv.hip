@@ -165,7 +165,20 @@ template <> struct IoVec<__hip_bfloat16, 4> { typedef unsigned short type __attr
 
 // Columns j .. j+3 of a row of length L (j a multiple of 4), clamped into
 // the row and unmasked: the caller zeroes columns >= L when it consumes them,
-// so the loads stay in flight until then.
+// so the loads stay in flight until then.  load_quad_vec keeps the quad as
+// one packed vector (2 VGPRs for bf16).
+template <class TIO, bool VL>
+__device__ __forceinline__ void load_quad_vec(const TIO *row, int j, int L, typename IoVec<TIO, 4>::type &out) {
+    using V4 = typename IoVec<TIO, 4>::type;
+    if constexpr (VL) {
+        out = *reinterpret_cast<const V4 *>(row + (j < L ? j : L - 4));
+    } else {
+        using E = typename std::conditional<sizeof(TIO) == 4, float, unsigned short>::type;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) out[c] = __builtin_bit_cast(E, row[j + c < L ? j + c : L - 1]);
+    }
+}
+
 template <class TIO, bool VL> __device__ __forceinline__ void load_quad_raw(const TIO *row, int j, int L, TIO (&out)[4]) {
     if constexpr (VL) {
         using V4 = typename IoVec<TIO, 4>::type;
@@ -234,7 +247,7 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
 #pragma unroll
     for (int r = 0; r < 16; ++r) bias_r[r] = bias[32 * wv + 8 * (r / 4) + 4 * (lane / 32) + (r % 4)];
 
-    TIO pf[TPT][8][4];
+    typename IoVec<TIO, 4>::type pf[TPT][8];  // raw quads of the next tile
     TIO ph[HPT];
     auto load_tile = [&](int tl) __attribute__((always_inline)) {
         const int b = tl / tiles, t = tl - b * tiles;
@@ -246,7 +259,7 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
             const int q = task % NQ, g = task / NQ;
             const int j = j0 + 4 * q;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) load_quad_raw<TIO, VL>(xb + (i64)(8 * g + i) * L, j, L, pf[u][i]);
+            for (int i = 0; i < 8; ++i) load_quad_vec<TIO, VL>(xb + (i64)(8 * g + i) * L, j, L, pf[u][i]);
         }
         const bool snap = tl == tile_end - 1 && t < tiles - 1;
 #pragma unroll
@@ -275,9 +288,9 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     if constexpr (std::is_same<TIO, __hip_bfloat16>::value) {
-                        hv[i] = in ? __builtin_bit_cast(__bf16, pf[u][i][c]) : (__bf16)0.0f;
+                        hv[i] = in ? __builtin_bit_cast(__bf16, io_elem<TIO>(pf[u][i], c)) : (__bf16)0.0f;
                     } else {
-                        const float f = in ? io_to_f32(pf[u][i][c]) : 0.0f;
+                        const float f = in ? io_to_f32(io_elem<TIO>(pf[u][i], c)) : 0.0f;
                         const __bf16 h = to_bf16(f);
                         hv[i] = h;
                         if constexpr (XLO) lv[i] = to_bf16(f - (float)h);
