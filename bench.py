@@ -251,8 +251,8 @@ def measure_conv(torch, B):
     del x64
     wf, bf = w.float(), b.float()
     for name, mode, io in (("f32", B.CONV_F32, B.F32), ("bf16_io_f32", B.CONV_BF16, B.F32),
-                           ("bf16x3_io_f32", B.CONV_BF16X3, B.F32), ("bf16_io_bf16", B.CONV_BF16, B.BF16),
-                           ("bf16x3_io_bf16", B.CONV_BF16X3, B.BF16)):
+                           ("bf16x3_io_f32", B.CONV_BF16X3, B.F32), ("bf16x6_io_f32", B.CONV_BF16X6, B.F32),
+                           ("bf16_io_bf16", B.CONV_BF16, B.BF16), ("bf16x3_io_bf16", B.CONV_BF16X3, B.BF16)):
         try:
             plan = B.Conv1dPlan(mode, io, Bn, C, L, K, wf, bf)
         except B.BkmaError as e:
@@ -281,7 +281,7 @@ def measure_conv(torch, B):
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
-        peak = 2500.0 if mode in (B.CONV_BF16, B.CONV_BF16X3) else 157.3
+        peak = 2500.0 if mode in (B.CONV_BF16, B.CONV_BF16X3, B.CONV_BF16X6) else 157.3
         res[name] = {"kernel": plan.name(), "ms": round(ms, 4), "TFLOPs": round(flop / ms / 1e9, 2),
                      "frac_of_dense_bf16_peak" if peak == 2500.0 else "frac_of_fp32_peak":
                          round(flop / ms / 1e9 / peak, 4),

@@ -267,6 +267,9 @@ void bkma_adv_batch_destroy(bkma_adv_batch *batch);
  *  BKMA_CONV_BF16  : io F32 or BF16, weights F32; bf16 MFMA, FP32 accumulate
  *  BKMA_CONV_BF16X3: io F32 or BF16, weights F32; split bf16 (hi*hi + hi*lo
  *                    + lo*hi) MFMA, FP32 accumulate
+ *  BKMA_CONV_BF16X6: io F32, weights F32; three-term bf16 split (x = x0+x1+x2,
+ *                    w = w0+w1+w2 exactly), the six products down to 2^-16
+ *                    on bf16 MFMA, FP32 accumulation: FP32-level accuracy
  *  BKMA_CONV_FP8   : io F32 or BF16, weights F32; OCP FP8 e4m3 MFMA, FP32
  *                    accumulate.  Weights are quantized per output channel
  *                    (scale = the power of two >= max|w[oc]| / 448); the
@@ -277,7 +280,8 @@ typedef enum bkma_conv_mode {
     BKMA_CONV_F32 = 1,
     BKMA_CONV_BF16 = 2,
     BKMA_CONV_BF16X3 = 3,
-    BKMA_CONV_FP8 = 4
+    BKMA_CONV_FP8 = 4,
+    BKMA_CONV_BF16X6 = 5
 } bkma_conv_mode;
 
 typedef struct bkma_conv1d_plan bkma_conv1d_plan;

@@ -27,7 +27,7 @@ BKMA_ERR_OUT_OF_MEMORY = 5
 BASIC_RANGE, ND_RANGE, ADAPTIVE_WG = 0, 1, 2
 MEM_LOCAL, MEM_GLOBAL = 0, 1
 F64, F32, BF16 = 0, 1, 2
-CONV_F64, CONV_F32, CONV_BF16, CONV_BF16X3, CONV_FP8 = 0, 1, 2, 3, 4
+CONV_F64, CONV_F32, CONV_BF16, CONV_BF16X3, CONV_FP8, CONV_BF16X6 = 0, 1, 2, 3, 4, 5
 
 
 class BkmaError(RuntimeError):

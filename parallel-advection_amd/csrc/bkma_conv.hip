@@ -368,6 +368,209 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
     }
 }
 
+// Three-term split W fragments: w = w0 + w1 + w2 exactly (each bf16 piece
+// takes the next 8 significant bits of the FP32 weight), in the layout of
+// conv_pack_weights_kernel, planes [3][C/32][KS][64][8].
+__global__ void conv_pack_weights3_kernel(const float *__restrict__ w, int C, int K, __bf16 *__restrict__ out) {
+    const int KS = C * K / 16;
+    const i64 plane = (i64)(C / 32) * KS * 64 * 8;
+    for (i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x; idx < plane; idx += (i64)gridDim.x * blockDim.x) {
+        const int e = (int)(idx % 8);
+        const int l = (int)((idx / 8) % 64);
+        const int s = (int)((idx / 512) % KS);
+        const int wv = (int)(idx / (512 * (i64)KS));
+        const int oc = 32 * wv + l % 32;
+        const int r = 16 * s + 8 * (l / 32) + e;
+        const int k = r / C, ic = r - k * C;
+        const float v = w[((i64)oc * C + ic) * K + k];
+        const __bf16 p0 = to_bf16(v);
+        const float r1 = v - (float)p0;  // exact
+        const __bf16 p1 = to_bf16(r1);
+        out[idx] = p0;
+        out[plane + idx] = p1;
+        out[2 * plane + idx] = to_bf16(r1 - (float)p1);
+    }
+}
+
+// FP32-accurate conv from bf16 MFMAs: x and w are split into three bf16
+// terms (x0 + x1 + x2 and w0 + w1 + w2 represent the FP32 values exactly)
+// and the six products down to 2^-16 relative are accumulated in FP32:
+// w0 x0 into one accumulator, w0 x1 + w1 x0 + w0 x2 + w1 x1 + w2 x0 into a
+// second, so their rounding errors stay 2^-8 below the leading term's.  The
+// W fragments (3 x 16 B per lane per k-step) stream from L2 one k-step ahead;
+// the three x planes are staged in LDS, double-buffered, as in
+// conv_fp8_kernel.  FP32 storage only.
+template <int C, int K, int TN, bool VL>
+__global__ __launch_bounds__(2 * C, 1) void conv_bf16x6_kernel(float *__restrict__ x, const float *__restrict__ halo,
+                                                             const bf16x8_t *__restrict__ wq,
+                                                             const float *__restrict__ bias, int L, int tiles,
+                                                             int total_tiles, int chunk) {
+    constexpr int NW = C / 32;
+    constexpr int NT = 64 * NW;
+    constexpr int KS = C * K / 16;
+    constexpr int JW = TN + K - 1;
+    constexpr int RS = C + 8;     // bf16 elements per LDS row
+    constexpr int BUF = JW * RS;  // one plane of one buffer
+    constexpr int NSUB = TN / 32;
+    constexpr int NQ = TN / 4;
+    constexpr int NTASK = (C / 8) * NQ;
+    static_assert(NTASK % NT == 0, "staging tasks must divide evenly");
+    constexpr int TPT = NTASK / NT;
+    static_assert(KS % TPT == 0, "k-steps must split evenly over the staging tasks");
+    constexpr int PH = KS / TPT;
+    constexpr int NH = C * (K - 1);
+    constexpr int HPT = (NH + NT - 1) / NT;
+    constexpr i64 WPLANE = (i64)NW * KS * 64;  // fragments per weight plane
+    using V4 = typename IoVec<float, 4>::type;
+
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __bf16 *xs = reinterpret_cast<__bf16 *>(lds_raw);  // [2][3][JW][RS]
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int Lout = L - K + 1;
+
+    int tile = blockIdx.x * chunk;
+    const int tile_end = min(tile + chunk, total_tiles);
+    if (tile >= tile_end) return;  // empty chunk (uniform over the work-group)
+
+    const bf16x8_t *wsrc = wq + ((i64)wv * KS) * 64 + lane;
+    V4 pf[8];
+    float ph[HPT];
+    auto load_task = [&](int tl, int u) __attribute__((always_inline)) {
+        const int b = tl / tiles, t = tl - b * tiles;
+        const int task = tid + u * NT;
+        const int q = task % NQ, g = task / NQ;
+        const float *xb = x + (i64)b * C * L;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) load_quad_vec<float, VL>(xb + (i64)(8 * g + i) * L, t * TN + 4 * q, L, pf[i]);
+    };
+    auto store_task = [&](int buf, int tl, int u) __attribute__((always_inline)) {
+        __bf16 *p0 = xs + (buf * 3) * BUF;
+        const int task = tid + u * NT;
+        const int q = task % NQ, g = task / NQ;
+        const int j = (tl % tiles) * TN + 4 * q;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool in = j + c < L;
+            bf16x8_t h0, h1, h2;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const float v = in ? pf[i][c] : 0.0f;
+                const __bf16 a = to_bf16(v);
+                const float r1 = v - (float)a;
+                const __bf16 bq = to_bf16(r1);
+                h0[i] = a;
+                h1[i] = bq;
+                h2[i] = to_bf16(r1 - (float)bq);
+            }
+            const int off = (4 * q + c) * RS + 8 * g;
+            *reinterpret_cast<bf16x8_t *>(p0 + off) = h0;
+            *reinterpret_cast<bf16x8_t *>(p0 + BUF + off) = h1;
+            *reinterpret_cast<bf16x8_t *>(p0 + 2 * BUF + off) = h2;
+        }
+    };
+    auto load_halo = [&](int tl) __attribute__((always_inline)) {
+        const int b = tl / tiles, t = tl - b * tiles;
+        const float *xb = x + (i64)b * C * L;
+        const bool snap = tl == tile_end - 1 && t < tiles - 1;
+#pragma unroll
+        for (int u = 0; u < HPT; ++u) {
+            const int e = min(tid + u * NT, NH - 1);
+            const int ic = e / (K - 1), kk = e - ic * (K - 1);
+            const int j = t * TN + TN + kk;
+            ph[u] = *(snap ? halo + (i64)blockIdx.x * NH + e : xb + (i64)ic * L + (j < L ? j : L - 1));
+        }
+    };
+    auto store_halo = [&](int buf, int tl) __attribute__((always_inline)) {
+        __bf16 *p0 = xs + (buf * 3) * BUF;
+        const int t = tl % tiles;
+        const bool snap = tl == tile_end - 1 && t < tiles - 1;
+#pragma unroll
+        for (int u = 0; u < HPT; ++u) {
+            const int e = tid + u * NT;
+            const int ic = e / (K - 1), kk = e - ic * (K - 1);
+            const float v = snap || t * TN + TN + kk < L ? ph[u] : 0.0f;
+            const __bf16 a = to_bf16(v);
+            const float r1 = v - (float)a;
+            const __bf16 bq = to_bf16(r1);
+            const int off = (TN + kk) * RS + ic;
+            if (e < NH) {
+                p0[off] = a;
+                p0[BUF + off] = bq;
+                p0[2 * BUF + off] = to_bf16(r1 - (float)bq);
+            }
+        }
+    };
+
+#pragma unroll
+    for (int u = 0; u < TPT; ++u) {
+        load_task(tile, u);
+        store_task(0, tile, u);
+    }
+    load_halo(tile);
+    store_halo(0, tile);
+    __syncthreads();
+    const int jl = lane % 32, kh = 8 * (lane / 32);
+    for (int it = 0; tile < tile_end; ++it) {
+        const int next = tile + 1;
+        const bool has_next = next < tile_end;
+        const __bf16 *p0 = xs + ((it & 1) * 3) * BUF;
+        f32x16_t acc[NSUB], acl[NSUB];
+#pragma unroll
+        for (int n = 0; n < NSUB; ++n)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[n][r] = acl[n][r] = 0.0f;
+        if (has_next) load_halo(next);
+        bf16x8_t a0 = wsrc[0], a1 = wsrc[WPLANE], a2 = wsrc[2 * WPLANE];
+        for (int u = 0; u < TPT; ++u) {
+            if (has_next) load_task(next, u);
+#pragma unroll 1
+            for (int ss = 0; ss < PH; ++ss) {
+                const int s = u * PH + ss;
+                const i64 sn = ((s + 1) % KS) * 64;  // next k-step (wraps to the next tile's first)
+                const bf16x8_t n0 = wsrc[sn], n1 = wsrc[WPLANE + sn], n2 = wsrc[2 * WPLANE + sn];
+                const int r0 = 16 * s;
+                const int k = r0 / C, ic0 = r0 - k * C + kh;
+#pragma unroll
+                for (int n = 0; n < NSUB; ++n) {
+                    const int off = (32 * n + jl + k) * RS + ic0;
+                    const bf16x8_t b0 = *reinterpret_cast<const bf16x8_t *>(p0 + off);
+                    const bf16x8_t b1 = *reinterpret_cast<const bf16x8_t *>(p0 + BUF + off);
+                    const bf16x8_t b2 = *reinterpret_cast<const bf16x8_t *>(p0 + 2 * BUF + off);
+                    acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[n], 0, 0, 0);
+                    acl[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acl[n], 0, 0, 0);
+                    acl[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acl[n], 0, 0, 0);
+                    acl[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b2, acl[n], 0, 0, 0);
+                    acl[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acl[n], 0, 0, 0);
+                    acl[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, b0, acl[n], 0, 0, 0);
+                }
+                a0 = n0;
+                a1 = n1;
+                a2 = n2;
+            }
+            if (has_next) store_task((it + 1) & 1, next, u);
+        }
+        if (has_next) store_halo((it + 1) & 1, next);
+        {
+            const int b = tile / tiles, t = tile - b * tiles;
+            float *xb = x + (i64)b * C * L;
+#pragma unroll
+            for (int n = 0; n < NSUB; ++n) {
+                const int j = t * TN + 32 * n + jl;
+                if (j < Lout) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int oc = 32 * wv + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+                        xb[(i64)oc * L + j] = (acc[n][r] + acl[n][r]) + bias[oc];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        tile = next;
+    }
+}
+
 // Plain 32x32x16 bf16 MFMA over one fragment pair, for the layout test:
 // a[l][e], b[l][e] are the lane fragments, d[l][r] the accumulator.
 __global__ void mfma_layout_probe_kernel(const __bf16 *a, const __bf16 *b, float *d) {
@@ -733,6 +936,40 @@ const MfmaVariant kFp8Variants[] = {
 };
 #undef FV
 
+template <int C, int K, int TN> void launch_bf16x6(void *x, const ConvPlan &p, hipStream_t s) {
+    const int tiles = (int)ceil_div((u64)(p.L - p.K + 1), (u64)TN);
+    const int total = (int)(p.B * tiles);
+    int grid = 0, chunk = 0;
+    launch_halo<float>(x, p, C, K, TN, tiles, total, &grid, &chunk, s);
+    auto k = p.L % 4 == 0 ? conv_bf16x6_kernel<C, K, TN, true> : conv_bf16x6_kernel<C, K, TN, false>;
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds_bytes);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(2 * C), (size_t)p.lds_bytes, s, reinterpret_cast<float *>(x),
+                       reinterpret_cast<const float *>(p.halo), (const bf16x8_t *)p.w, (const float *)p.bias,
+                       (int)p.L, tiles, total, chunk);
+}
+
+template <int C, int K, int TN> int occupancy_bf16x6(u64 lds_bytes) {
+    auto k = conv_bf16x6_kernel<C, K, TN, true>;
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 2 * C, (size_t)lds_bytes) != hipSuccess) return 0;
+    return blocks;
+}
+
+#define XV(C, K, TN) {BKMA_F32, 6, C, K, TN, &launch_bf16x6<C, K, TN>, &occupancy_bf16x6<C, K, TN>}
+const MfmaVariant kBf16x6Variants[] = {
+    XV(128, 5, 64),
+    XV(128, 3, 64),
+    XV(64, 5, 64),
+};
+#undef XV
+
+static const MfmaVariant *find_bf16x6(bkma_dtype io, i64 C, i64 K) {
+    for (const auto &v : kBf16x6Variants)
+        if (v.io == io && v.C == C && v.K == K) return &v;
+    return nullptr;
+}
+
 static const MfmaVariant *find_fp8(bkma_dtype io, i64 C, i64 K) {
     for (const auto &v : kFp8Variants)
         if (v.io == io && v.C == C && v.K == K) return &v;
@@ -818,6 +1055,29 @@ extern "C" bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype i
         p->name = std::string("conv_mfma_bf16") + (nprod == 3 ? "x3" : "") + "<C" + std::to_string(C) + ",K" +
                   std::to_string(K) + ",TN" + std::to_string(mv->TN) + (io_dtype == BKMA_BF16 ? ",io_bf16" : ",io_f32") +
                   ">";
+    } else if (mode == BKMA_CONV_BF16X6) {
+        const MfmaVariant *mv = find_bf16x6(io_dtype, (i64)C, (i64)K);
+        if (!mv)
+            return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: no BF16X6 kernel compiled for C=" + std::to_string(C) +
+                                                       ", K=" + std::to_string(K) + " and this io dtype (F32 only)"));
+        if (w_dtype != BKMA_F32) return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: BF16X6 takes F32 weights"));
+        if (hipMalloc(&p->w, (size_t)nw * 2 * 3) != hipSuccess || hipMalloc(&p->bias, C * 4) != hipSuccess)
+            return bail(fail(BKMA_ERR_OUT_OF_MEMORY, "conv1d plan: weight fragments"));
+        hipLaunchKernelGGL(conv_pack_weights3_kernel, dim3(256), dim3(256), 0, s, (const float *)weight, (int)C, (int)K,
+                           (__bf16 *)p->w);
+        if (hipMemcpyAsync(p->bias, bias, C * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            return bail(fail(BKMA_ERR_DEVICE, "conv1d plan: bias copy"));
+        p->lds_bytes = 2ull * 3 * (u64)(mv->TN + K - 1) * (u64)(C + 8) * 2;
+        if (p->lds_bytes > lds_max) return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: LDS tile too large"));
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return bail(fail(BKMA_ERR_DEVICE, "conv1d plan: device query"));
+        p->resident_groups = (i64)std::max(1, mv->occ(p->lds_bytes)) * cus;
+        if (hipMalloc(&p->halo, std::max<u64>((u64)p->resident_groups * C * (K - 1) * 4, 16)) != hipSuccess)
+            return bail(fail(BKMA_ERR_OUT_OF_MEMORY, "conv1d plan: halo buffer"));
+        p->name = std::string("conv_mfma_bf16x6<C") + std::to_string(C) + ",K" + std::to_string(K) + ",TN" +
+                  std::to_string(mv->TN) + ",io_f32>";
     } else if (mode == BKMA_CONV_FP8) {
         const MfmaVariant *mv = find_fp8(io_dtype, (i64)C, (i64)K);
         if (!mv)
@@ -875,6 +1135,12 @@ extern "C" bkma_status bkma_conv1d_plan_run(bkma_conv1d_plan *p, void *x, void *
         case BKMA_CONV_BF16X3: {
             const MfmaVariant *v = find_mfma(p->io, p->mode == BKMA_CONV_BF16 ? 1 : 3, p->C, p->K);
             if (!v) return fail(BKMA_ERR_UNSUPPORTED, "conv1d run: no MFMA kernel");
+            v->fn(x, *p, s);
+            break;
+        }
+        case BKMA_CONV_BF16X6: {
+            const MfmaVariant *v = find_bf16x6(p->io, p->C, p->K);
+            if (!v) return fail(BKMA_ERR_UNSUPPORTED, "conv1d run: no BF16X6 kernel");
             v->fn(x, *p, s);
             break;
         }

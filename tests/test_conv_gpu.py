@@ -74,10 +74,13 @@ def _rel(got, ref, Lout):
 
 # bf16 storage rounds every output to 8 bits: rel_max is then bounded by the
 # output rounding (~2^-8 of the largest output), whatever the products' precision.
+# bf16x3 (two-term split, three products) carries 16 significant bits; bf16x6
+# (three-term split, six products) represents x and w exactly in FP32 and
+# meets 1e-6.
 @pytest.mark.parametrize("mode,io,tol", [(B.CONV_F32, B.F32, 1e-6), (B.CONV_BF16, B.F32, 1e-2),
                                          (B.CONV_BF16X3, B.F32, 1e-4), (B.CONV_BF16, B.BF16, 1e-2),
-                                         (B.CONV_BF16X3, B.BF16, 8e-3)],
-                         ids=["f32", "bf16_io32", "bf16x3_io32", "bf16_io16", "bf16x3_io16"])
+                                         (B.CONV_BF16X3, B.BF16, 8e-3), (B.CONV_BF16X6, B.F32, 1e-6)],
+                         ids=["f32", "bf16_io32", "bf16x3_io32", "bf16_io16", "bf16x3_io16", "bf16x6_io32"])
 @pytest.mark.parametrize("K", [5, 3])
 @pytest.mark.parametrize("L", [517, 516], ids=["L517", "L516"])  # scalar and vector staging loads
 def test_conv_reduced_precision(cuda, mode, io, tol, K, L):
@@ -101,6 +104,14 @@ def test_conv_reduced_precision(cuda, mode, io, tol, K, L):
     print(f"rel_max {rmax:.3e} rel_L2 {rl2:.3e}", flush=True)
     assert rmax <= tol
     assert np.array_equal(got[:, :, Lout:], x_in[:, :, Lout:])  # tail kept
+
+
+def test_conv_bf16x6_needs_f32_storage(cuda):
+    torch = _torch()
+    w = torch.zeros((128, 128, 5), dtype=torch.float32, device="cuda")
+    b = torch.zeros(128, dtype=torch.float32, device="cuda")
+    with pytest.raises(B.BkmaUnsupported):
+        B.Conv1dPlan(B.CONV_BF16X6, B.BF16, 2, 128, 100, 5, w, b)
 
 
 def test_conv_unsupported_shape_is_reported(cuda):
@@ -231,7 +242,8 @@ def test_conv_fp8_matches_quantized_emulation(cuda, io):
 # More tiles than resident work-groups: the persistent loop, the prefetch of
 # the next tile into the other LDS buffer and the halo snapshots all run.
 @pytest.mark.parametrize("mode,io,tol", [(B.CONV_BF16, B.F32, 1e-2), (B.CONV_BF16X3, B.F32, 1e-4),
-                                         (B.CONV_BF16, B.BF16, 1e-2)], ids=["bf16_io32", "bf16x3_io32", "bf16_io16"])
+                                         (B.CONV_BF16, B.BF16, 1e-2), (B.CONV_BF16X6, B.F32, 1e-6)],
+                         ids=["bf16_io32", "bf16x3_io32", "bf16_io16", "bf16x6_io32"])
 def test_conv_mfma_persistent_many_tiles(cuda, mode, io, tol):
     torch = _torch()
     Bn, C, L, K = 40, 128, 1024, 5
