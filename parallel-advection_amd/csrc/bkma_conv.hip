@@ -244,6 +244,10 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
         }
     }
 
+    float bias_r[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) bias_r[r] = bias[32 * wv + 8 * (r / 4) + 4 * (lane / 32) + (r % 4)];
+
     typename IoVec<TIO, 4>::type pf[TPT][8];  // raw quads of the next tile
     TIO ph[HPT];
     auto load_tile = [&](int tl) __attribute__((always_inline)) {
@@ -357,7 +361,7 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
                         const int oc = 32 * wv + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
-                        xb[(i64)oc * L + j] = f32_to_io<TIO>(acc[n][r] + bias[oc]);  // bias: L1-resident
+                        xb[(i64)oc * L + j] = f32_to_io<TIO>(acc[n][r] + bias_r[r]);
                     }
                 }
             }
