@@ -217,3 +217,43 @@ def test_autotune_small_n2_long_line(cuda):
     ref = host.copy()
     O.advect_f32_blocked(ref, po, 2, spp)
     assert np.array_equal(got, ref)
+
+
+def test_tuned_c0_config_bit_exact(cuda):
+    # [256][1024][1] FP64, sin(2 pi x) + 0.1 U(0,1) (seed 0), 100 steps: the
+    # tuned plan (whole lines in LDS) against the oracle's reference arithmetic
+    n0, n1, n2 = 256, 1024, 1
+    p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    po = O.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    x = (np.arange(n1) / n1)[None, :, None]
+    host = np.sin(2 * np.pi * x) + 0.1 * O.uniform(n0 * n1 * n2, 0).reshape(n0, n1, n2)
+    got, _ = _run(host, p, B.F64, 100)
+    ref = host.copy()
+    O.advect_f64(ref, po, 100)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dtype", [B.F64, B.F32], ids=["f64", "f32"])
+def test_zero_steps_is_identity(cuda, dtype):
+    p = B.make_params(4, 1024, 512, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    npdt = np.float64 if dtype == B.F64 else np.float32
+    host = O.uniform(4 * 1024 * 512, 9, npdt).reshape(4, 1024, 512)
+    got, _ = _run(host, p, dtype, 0)
+    assert np.array_equal(got, host)
+
+
+@pytest.mark.parametrize("shape", [(1, 32, 1), (1, 64, 64), (2, 40, 7), (1, 1024, 4096)])
+@pytest.mark.parametrize("dtype", [B.F64, B.F32], ids=["f64", "f32"])
+def test_small_and_single_slab_shapes(cuda, shape, dtype):
+    n0, n1, n2 = shape
+    p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    po = O.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    npdt = np.float64 if dtype == B.F64 else np.float32
+    host = O.uniform(n0 * n1 * n2, 10, npdt).reshape(shape)
+    got, spp = _run(host, p, dtype, 5)
+    ref = host.copy()
+    if dtype == B.F64:
+        O.advect_f64(ref, po, 5)
+    else:
+        O.advect_f32_blocked(ref, po, 5, spp)
+    assert np.array_equal(got, ref)
