@@ -36,6 +36,11 @@
 namespace bkma {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+#ifndef BKMA_CONV_PAD
+#define BKMA_CONV_PAD 8
+#endif
+constexpr int kConvPad = BKMA_CONV_PAD;  // bf16 elements of padding per LDS row (keeps 16-byte alignment)
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ __bf16 to_bf16(float f) { return (__bf16)f; }  // round to nearest even
@@ -197,7 +202,8 @@ template <class TIO, bool VL> __device__ __forceinline__ void load_quad_raw(cons
 // a grid-stride loop, double-buffered in LDS: the next tile's global loads
 // are issued before the current tile's MFMAs and written to the other LDS
 // buffer after them, one barrier per tile.
-// LDS tile: Xs[jj][ic] bf16, row stride C + 8 (16-byte pad).
+// LDS tile: Xs[jj][ic] bf16, row stride C + kConvPad (padding against bank
+// conflicts of the B-fragment reads).
 template <class TIO, int NPROD, int C, int K, int TN, bool VL>
 __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x, const TIO *__restrict__ halo,
                                                            const __bf16 *__restrict__ whi,
@@ -209,7 +215,7 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
     constexpr int KS = C * K / 16;
     constexpr bool XLO = NPROD == 3 && !std::is_same<TIO, __hip_bfloat16>::value;  // bf16 input: x = xh exactly
     constexpr int JW = TN + K - 1;
-    constexpr int RS = C + 8;
+    constexpr int RS = C + kConvPad;
     constexpr int NPART = XLO ? 2 : 1;
     constexpr int BUF = JW * RS;  // bf16 elements of one part of one buffer
     constexpr int NSUB = TN / 32;
@@ -413,7 +419,7 @@ __global__ __launch_bounds__(2 * C, 1) void conv_bf16x6_kernel(float *__restrict
     constexpr int NT = 64 * NW;
     constexpr int KS = C * K / 16;
     constexpr int JW = TN + K - 1;
-    constexpr int RS = C + 8;     // bf16 elements per LDS row
+    constexpr int RS = C + kConvPad;  // bf16 elements per LDS row
     constexpr int BUF = JW * RS;  // one plane of one buffer
     constexpr int NSUB = TN / 32;
     constexpr int NQ = TN / 4;
@@ -1046,7 +1052,7 @@ extern "C" bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype i
         if (hipMemcpyAsync(p->bias, bias, C * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
             return bail(fail(BKMA_ERR_DEVICE, "conv1d plan: bias copy"));
         const bool xlo = nprod == 3 && io_dtype != BKMA_BF16;
-        p->lds_bytes = 2ull * (u64)(mv->TN + K - 1) * (u64)(C + 8) * 2 * (xlo ? 2 : 1);
+        p->lds_bytes = 2ull * (u64)(mv->TN + K - 1) * (u64)(C + kConvPad) * 2 * (xlo ? 2 : 1);
         if (p->lds_bytes > lds_max) return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: LDS tile too large"));
         int dev = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
@@ -1071,7 +1077,7 @@ extern "C" bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype i
                            (__bf16 *)p->w);
         if (hipMemcpyAsync(p->bias, bias, C * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
             return bail(fail(BKMA_ERR_DEVICE, "conv1d plan: bias copy"));
-        p->lds_bytes = 2ull * 3 * (u64)(mv->TN + K - 1) * (u64)(C + 8) * 2;
+        p->lds_bytes = 2ull * 3 * (u64)(mv->TN + K - 1) * (u64)(C + kConvPad) * 2;
         if (p->lds_bytes > lds_max) return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: LDS tile too large"));
         int dev = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
