@@ -183,7 +183,9 @@ bkma_status bkma_conv_run(bkma_impl impl, bkma_dtype dtype, void *data, uint64_t
  *    of the remaining steps. */
 typedef enum bkma_strategy {
     BKMA_STRATEGY_COLUMN_STREAM = 1, /* lines strided by n2 (n2 >= 64), small shifts: register ring per wavefront */
-    BKMA_STRATEGY_LINE_LDS = 2       /* any n2 and shift: slab segments + halos staged in LDS */
+    BKMA_STRATEGY_LINE_LDS = 2,      /* any n2 and shift: slab segments + halos staged in LDS */
+    BKMA_STRATEGY_COLUMN_PAIR = 3    /* as COLUMN_STREAM, for slabs whose rows take at most two adjacent shifts
+                                        (checked per slab at plan creation): seven-row windows, no shift branches */
 } bkma_strategy;
 
 typedef struct bkma_tuned_config {

@@ -1,0 +1,253 @@
+// Column-pair advection kernel: the column stream of adv_stream.hpp for
+// slabs whose backward shifts d = i1 - leftNode take at most two adjacent
+// values {D, D+1} along the whole line.
+//
+// With a constant velocity per i0 slab the foot of row i1 lies s = v*dt/dx
+// cells back, so leftNode = floor(i1 - s + e) with e a rounding error far
+// below one cell: d = ceil(s - e) is the same for every row unless s is
+// within e of an integer, and then it takes two adjacent values.  A plan
+// scans every (i0, i1) once with the device's own foot arithmetic
+// (slab_shift_scan_kernel) and keeps D = min d of each slab.
+//
+// The six taps of a row with shift d read rows i1-d-2 .. i1-d+3.  Over the
+// seven rows i1-D-3 .. i1-D+3 they are the seven coefficients
+//   d = D+1: (c0, c1, c2, c3, c4, c5, 0)
+//   d = D  : (0, c0, c1, c2, c3, c4, c5)
+// The extra product is an exact zero, so the separately rounded sum (FP64
+// storage) and the fused sum (FP32 storage) keep the reference's value bit
+// for bit on finite data, and the inner loop has no branch on the shift.
+// One instance of the stream loop per D (selected once per work unit) fixes
+// the stage lag (3 - D rows) and the window (seven rows per stage) at
+// compile time.  Periodic wrap, in-place order, the head rows kept in LDS,
+// the foot table and the FP64 intermediate stages follow adv_stream.hpp.
+#pragma once
+
+#include "adv_stream.hpp"
+
+namespace bkma {
+
+constexpr int kPairTaps = 7;
+
+// Stage windows for a slab whose shifts are {D, D+1}.
+template <int D, int P, int NS> struct PairShape {
+    static constexpr int BACK = D + 3;   // rows behind the output row read by its taps
+    static constexpr int AHEAD = 3 - D;  // rows in front of it (stage lag)
+    static constexpr int W = kPairTaps;  // rows per stage window
+    static constexpr int U = clcm(W, P);
+    static constexpr int R = NS * BACK;
+    static constexpr int H = NS * AHEAD;
+    static constexpr int TB = NS * AHEAD;
+    static_assert(BACK >= 0 && AHEAD >= 1, "shift pair outside the supported range");
+    static_assert(U + TB <= 64, "foot table must fit one wavefront");
+};
+
+// Seven-tap sum of VEC cells from window slots S0..S0+6 (mod W), in FP64:
+// separately rounded multiply and add for FP64 storage, fused for FP32.
+template <class T, class WT, class OT, int VEC, int S0>
+__device__ __forceinline__ void pair_taps(const WT (&win)[kPairTaps][VEC], const double (&c)[kPairTaps],
+                                          OT (&out)[VEC]) {
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+        double acc = 0.;
+        if constexpr (sizeof(T) == 8) {
+#pragma unroll
+            for (int k = 0; k < kPairTaps; ++k) acc = acc + c[k] * (double)win[(S0 + k) % kPairTaps][v];
+        } else {
+#pragma unroll
+            for (int k = 0; k < kPairTaps; ++k) acc = fma(c[k], (double)win[(S0 + k) % kPairTaps][v], acc);
+        }
+        out[v] = (OT)acc;
+    }
+}
+
+// One wavefront advances column tile `tile` of slab i0 by NS steps; every
+// row's shift is D or D+1.
+template <class T, class WT, int VEC, int P, int NS, bool NT, int D>
+__device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs &a, i64 i0, i64 tile, int lane,
+                                          T (*head)[64 * VEC]) {
+    using S = PairShape<D, P, NS>;
+    constexpr int AHEAD = S::AHEAD, BACK = S::BACK, W = S::W, U = S::U, R = S::R, H = S::H, TB = S::TB;
+    constexpr int NM = NS > 1 ? NS - 1 : 1;
+    using M = VecMem<T, VEC, NT>;
+    const AdvDev &p = a.p;
+    const i64 n1 = p.n1, n2 = p.n2;
+    const i64 col = (tile * 64 + lane) * VEC;
+    const bool lane_ok = col < n2;  // VEC divides n2: the whole vector is in range
+    T *slab = data + (i0 * n1) * n2 + (lane_ok ? col : 0);
+    const i64 first_end = R + n1;  // end of the positions read from HBM
+    const i64 total = first_end + H;
+
+    WT w0[W][VEC];
+    double wm[NM][W][VEC];
+    T stg[P][VEC];
+#pragma unroll
+    for (int s = 0; s < W; ++s)
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            w0[s][v] = (WT)0;
+#pragma unroll
+            for (int t = 0; t < NM; ++t) wm[t][s][v] = 0.;
+        }
+
+    auto load_pos = [&](i64 pos, T (&dst)[VEC]) __attribute__((always_inline)) {
+        const i64 row = pos < R ? pos - R + n1 : pos - R;  // pos < first_end
+        if (lane_ok) {
+            M::load(slab + row * n2, dst);
+        } else {
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) dst[v] = (T)0;
+        }
+    };
+    static_for<0, P>([&](auto sc) __attribute__((always_inline)) {
+        constexpr int s = decltype(sc)::value;
+        load_pos(s, stg[s]);
+    });
+
+    const int half = (int)(n1 / 2);
+    double tab[kPairTaps];
+
+    for (i64 base = 0; base < total; base += U) {
+        // Foot table: lane l holds the seven coefficients of the row at
+        // position base - TB + l.
+        {
+            i64 row = base - TB + lane - R;
+            if (row < 0) row += n1;
+            if (row >= n1) row -= n1;
+            if (row < 0 || row >= n1) row = 0;  // positions no stored value depends on
+            double d_prev1, c[6];
+            const int leftNode = adv_foot(p, (int)i0, (int)row, d_prev1);
+            adv_lag_basis(d_prev1, c);
+            int d = (int)row - leftNode;
+            if (d > half) d -= (int)n1;
+            if (d < -half) d += (int)n1;
+            const bool late = d == D;  // taps start one row later in the window
+            tab[0] = late ? 0. : c[0];
+#pragma unroll
+            for (int k = 1; k < 6; ++k) tab[k] = late ? c[k - 1] : c[k];
+            tab[6] = late ? c[5] : 0.;
+        }
+
+        auto step = [&](auto jc) __attribute__((always_inline)) {
+            constexpr int J = decltype(jc)::value;
+            const i64 pin = base + J;
+            if (pin >= total) return;
+            // 1. position pin enters the stage-0 window
+            {
+                constexpr int sw = J % W, ss = J % P;
+                if (pin < first_end) {
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) w0[sw][v] = (WT)stg[ss][v];
+                    if (pin >= R && pin < R + H) {
+#pragma unroll
+                        for (int v = 0; v < VEC; ++v) head[pin - R][lane * VEC + v] = stg[ss][v];
+                    }
+                    // 2. refill the staging slot with position pin + P
+                    if (pin + P < first_end) load_pos(pin + P, stg[ss]);
+                } else {
+                    const int h = (int)(pin - first_end);
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) w0[sw][v] = (WT)head[h][lane * VEC + v];
+                }
+            }
+            // 3. stage t computes position pin - t*AHEAD from the seven rows
+            //    around it in stage t-1
+            static_for<1, NS + 1>([&](auto tc) __attribute__((always_inline)) {
+                constexpr int t = decltype(tc)::value;
+                constexpr int OFF = J - t * AHEAD;
+                constexpr int L = OFF + TB;
+                constexpr int S0 = cmod(OFF - BACK, W);
+                double c[kPairTaps];
+#pragma unroll
+                for (int k = 0; k < kPairTaps; ++k) c[k] = lane_value<L>(tab[k]);
+                if constexpr (t < NS) {
+                    double o[VEC];
+                    if constexpr (t == 1)
+                        pair_taps<T, WT, double, VEC, S0>(w0, c, o);
+                    else
+                        pair_taps<T, double, double, VEC, S0>(wm[t - 2], c, o);
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) wm[t - 1][cmod(OFF, W)][v] = o[v];
+                } else {
+                    T o[VEC];
+                    if constexpr (NS == 1)
+                        pair_taps<T, WT, T, VEC, S0>(w0, c, o);
+                    else
+                        pair_taps<T, double, T, VEC, S0>(wm[NS - 2], c, o);
+                    const i64 pt = pin - NS * AHEAD;
+                    if (pt >= R && pt < first_end && lane_ok) M::store(slab + (pt - R) * n2, o);
+                }
+            });
+        };
+        static_for<0, U>(step);
+    }
+}
+
+// Selects the instance of D (wave-uniform) in [DLO, DHI].
+template <class T, class WT, int VEC, int P, int NS, bool NT, int DLO, int DHI>
+__device__ __forceinline__ void pair_dispatch(int d, T *__restrict__ data, const StreamArgs &a, i64 i0, i64 tile,
+                                              int lane, T (*head)[64 * VEC]) {
+    if constexpr (DLO <= DHI) {
+        if (d == DLO) {
+            pair_unit<T, WT, VEC, P, NS, NT, DLO>(data, a, i0, tile, lane, head);
+            return;
+        }
+        pair_dispatch<T, WT, VEC, P, NS, NT, DLO + 1, DHI>(d, data, a, i0, tile, lane, head);
+    }
+}
+
+// Slabs with shifts in [-SMAX, SMAX]: D in [-SMAX, SMAX-1] (a slab whose
+// only shift is SMAX runs as the pair {SMAX-1, SMAX}).
+template <class T, class WT, int VEC, int SMAX, int P, int NS, int WPB, bool NT>
+__global__ __launch_bounds__(64 * WPB) void adv_column_pair_kernel(T *__restrict__ data, StreamArgs a) {
+    __shared__ T head_lds[WPB][NS * (SMAX + 3)][64 * VEC];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    for (i64 unit = (i64)blockIdx.x * WPB + wv; unit < a.total_units; unit += a.wave_stride) {
+        const i64 i0 = unit / a.tiles_per_slab;
+        const i64 tile = unit - i0 * a.tiles_per_slab;
+        const int d = wave_uniform(min(a.slab_dlo[i0], SMAX - 1));
+        pair_dispatch<T, WT, VEC, P, NS, NT, -SMAX, SMAX - 1>(d, data, a, i0, tile, lane, head_lds[wv]);
+    }
+}
+
+// Per-slab shift range with the kernels' own foot arithmetic: dlo[i0] =
+// min d, and stats = {min over slabs of min d, max over slabs of max d,
+// max over slabs of (max d - min d)} (stats preset to {INT_MAX, INT_MIN, 0}).
+__global__ void slab_shift_scan_kernel(AdvDev p, int *__restrict__ dlo, int *__restrict__ stats);
+
+// ---------------------------------------------------------------------------
+// Host side: launchers for steps-per-pass 1 .. NSMAX of one variant
+// ---------------------------------------------------------------------------
+template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, int NS>
+void launch_pair(void *data, const StreamArgs &a, unsigned grid, hipStream_t s, int ns) {
+    if constexpr (NS >= 1) {
+        if (ns == NS) {
+            hipLaunchKernelGGL((adv_column_pair_kernel<T, WT, VEC, SMAX, P, NS, WPB, NT>), dim3(grid), dim3(64 * WPB),
+                               0, s, reinterpret_cast<T *>(data), a);
+            return;
+        }
+        launch_pair<T, WT, VEC, SMAX, P, WPB, NT, NS - 1>(data, a, grid, s, ns);
+    }
+}
+
+template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, int NS> int occupancy_pair(int ns) {
+    if constexpr (NS >= 1) {
+        if (ns == NS) {
+            int blocks = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &blocks, adv_column_pair_kernel<T, WT, VEC, SMAX, P, NS, WPB, NT>, 64 * WPB, 0) != hipSuccess)
+                return 0;
+            return blocks;
+        }
+        return occupancy_pair<T, WT, VEC, SMAX, P, WPB, NT, NS - 1>(ns);
+    }
+    return 0;
+}
+
+#define BKMA_PAIR_VARIANT(T, WT, DT, V, S, P, W, NT, NSMAX)                                                   \
+    StreamVariant {                                                                                          \
+        DT, 1, V, S, P, W, NT, sizeof(WT) > sizeof(T), NSMAX, &launch_pair<T, WT, V, S, P, W, NT, NSMAX>,    \
+            &occupancy_pair<T, WT, V, S, P, W, NT, NSMAX>                                                    \
+    }
+
+}  // namespace bkma

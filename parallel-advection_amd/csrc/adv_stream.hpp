@@ -137,9 +137,10 @@ __device__ __forceinline__ void interp_shift(int d, const WT (&win)[WS][VEC], co
 
 struct StreamArgs {
     AdvDev p;
-    i64 tiles_per_slab;  // column tiles of 64*VEC per i0 slab
-    i64 total_units;     // n0 * tiles_per_slab
-    i64 wave_stride;     // waves in the grid (persistent loop stride)
+    i64 tiles_per_slab;   // column tiles of 64*VEC per i0 slab
+    i64 total_units;      // n0 * tiles_per_slab
+    i64 wave_stride;      // waves in the grid (persistent loop stride)
+    const int *slab_dlo;  // column-pair kernels: smallest shift of each i0 slab
 };
 
 // One wavefront advances unit (i0, column tile) by NS steps.
@@ -309,19 +310,20 @@ template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, int NS>
 
 struct StreamVariant {
     bkma_dtype dtype;
+    int pair;                                // 0: shift ring (this file), 1: per-slab shift pair (adv_pair.hpp)
     int vec, smax, prefetch, wpb, nt, wide;  // wide: FP64 stage-0 window for FP32 storage
     int nsmax;                               // steps per pass compiled: 1 .. nsmax
     StreamLaunchFn fn;
     StreamOccFn occ;
 };
 
-#define BKMA_STREAM_VARIANT(T, WT, DT, V, S, P, W, NT, NSMAX)                                               \
-    StreamVariant {                                                                                        \
-        DT, V, S, P, W, NT, sizeof(WT) > sizeof(T), NSMAX, &launch_stream<T, WT, V, S, P, W, NT, NSMAX>,   \
-            &occupancy_stream<T, WT, V, S, P, W, NT, NSMAX>                                                \
+#define BKMA_STREAM_VARIANT(T, WT, DT, V, S, P, W, NT, NSMAX)                                                  \
+    StreamVariant {                                                                                           \
+        DT, 0, V, S, P, W, NT, sizeof(WT) > sizeof(T), NSMAX, &launch_stream<T, WT, V, S, P, W, NT, NSMAX>,   \
+            &occupancy_stream<T, WT, V, S, P, W, NT, NSMAX>                                                   \
     }
 
-// The compiled variants of each storage type (host functions).
+// The compiled variants of each storage type (host functions), both kernels.
 const StreamVariant *stream_variants_f32(int *count);
 const StreamVariant *stream_variants_f64(int *count);
 

@@ -4,6 +4,8 @@
 // 64*VEC consecutive i2 columns of one i0 slab and walks the whole line in
 // increasing i1 with a register window per fused time step (adv_stream.hpp);
 // a plan runs `steps_per_pass` steps per pass over HBM.
+// Strategy COLUMN_PAIR: the column stream for slabs whose rows take at most
+// two adjacent shifts (adv_pair.hpp): seven-tap windows, no shift branches.
 // Strategy LINE_LDS (any n2, any shift): slabs or segments of them staged in
 // LDS (adv_lds.hpp), one step per pass.
 //
@@ -18,11 +20,46 @@
 #include <string>
 #include <vector>
 
+#include <climits>
+
 #include "adv_lds.hpp"
+#include "adv_pair.hpp"
 #include "adv_stream.hpp"
 #include "common.hpp"
 
 namespace bkma {
+
+__global__ void slab_shift_scan_kernel(AdvDev p, int *__restrict__ dlo, int *__restrict__ stats) {
+    __shared__ int smin, smax;
+    const int half = (int)(p.n1 / 2);
+    for (i64 i0 = blockIdx.x; i0 < p.n0; i0 += gridDim.x) {
+        if (threadIdx.x == 0) {
+            smin = INT_MAX;
+            smax = INT_MIN;
+        }
+        __syncthreads();
+        int lo = INT_MAX, hi = INT_MIN;
+        for (i64 row = threadIdx.x; row < p.n1; row += blockDim.x) {
+            double d_prev1;
+            const int leftNode = adv_foot(p, (int)i0, (int)row, d_prev1);
+            int d = (int)row - leftNode;
+            if (d > half) d -= (int)p.n1;
+            if (d < -half) d += (int)p.n1;
+            lo = min(lo, d);
+            hi = max(hi, d);
+        }
+        atomicMin(&smin, lo);
+        atomicMax(&smax, hi);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            dlo[i0] = smin;
+            atomicMin(&stats[0], smin);
+            atomicMax(&stats[1], smax);
+            atomicMax(&stats[2], smax - smin);
+        }
+        __syncthreads();
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Host side: variant table, plan, autotuner
@@ -67,16 +104,18 @@ struct TunedPlan {
     bkma_dtype dtype;
     bkma_tuned_config cfg;
     std::string name;
-    // COLUMN_STREAM
+    // COLUMN_STREAM / COLUMN_PAIR
     const StreamVariant *variant = nullptr;
     int ns = 1;                        // steps per pass
     i64 persistent_blocks[9] = {};    // by steps per pass: 0 = one wave per unit, else the grid cap
+    int *slab_dlo = nullptr;           // COLUMN_PAIR: smallest shift of each slab (owned)
     // LINE_LDS
     LdsArgs lds{};
     u64 lds_bytes = 0;
     void *halo = nullptr;  // segmented mode: halo snapshot buffer (owned)
     ~TunedPlan() {
         if (halo) (void)hipFree(halo);
+        if (slab_dlo) (void)hipFree(slab_dlo);
     }
 };
 
@@ -108,9 +147,71 @@ bool shift_supported(const bkma_adv_params &p, std::string *why) {
     return true;
 }
 
+// Shifts of every (i0, i1) from the device's foot arithmetic: stats =
+// {smallest shift, largest shift, largest spread within one slab}; the
+// per-slab smallest shifts go to *dlo_out (device memory, owned by the
+// caller) when it is not null.
+bkma_status scan_slab_shifts(const bkma_adv_params &p, int **dlo_out, int stats[3]) {
+    int *dlo = nullptr, *dst = nullptr;
+    const int init[3] = {INT_MAX, INT_MIN, 0};
+    if (hipMalloc(&dlo, std::max<u64>(p.n0, 1) * sizeof(int)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(BKMA_ERR_OUT_OF_MEMORY, "tuned advection: slab shift table allocation failed");
+    }
+    if (hipMalloc(&dst, sizeof(init)) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(dlo);
+        return fail(BKMA_ERR_OUT_OF_MEMORY, "tuned advection: slab shift table allocation failed");
+    }
+    bkma_status st = BKMA_OK;
+    if (hipMemcpy(dst, init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) st = fail(BKMA_ERR_RUNTIME, "hipMemcpy");
+    if (st == BKMA_OK) {
+        const unsigned grid = (unsigned)std::min<u64>(p.n0, 65536);
+        hipLaunchKernelGGL(slab_shift_scan_kernel, dim3(grid), dim3(256), 0, 0, make_adv_dev(p), dlo, dst);
+        st = check_launch("slab_shift_scan_kernel");
+    }
+    if (st == BKMA_OK && hipMemcpy(stats, dst, sizeof(init), hipMemcpyDeviceToHost) != hipSuccess)
+        st = fail(BKMA_ERR_RUNTIME, "hipMemcpy");
+    (void)hipFree(dst);
+    if (st == BKMA_OK && dlo_out) {
+        *dlo_out = dlo;
+    } else {
+        (void)hipFree(dlo);
+    }
+    return st;
+}
+
+// Column-pair feasibility given the slab shift statistics.
+bkma_status pair_feasible(const bkma_adv_params &p, bkma_dtype dt, const StreamVariant &v, int ns,
+                          const int stats[3], std::string *why) {
+    if (v.dtype != dt || !v.pair) return BKMA_ERR_UNSUPPORTED;
+    if (ns < 1 || ns > v.nsmax) {
+        if (why) *why = "steps per pass not compiled for this variant";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    if (!shift_supported(p, why)) return BKMA_ERR_UNSUPPORTED;
+    if (p.n2 % (u64)v.vec != 0) {
+        if (why) *why = "n2 not a multiple of VEC";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    if (stats[2] > 1) {
+        if (why) *why = "a slab takes more than two shifts";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    if (stats[0] < -v.smax || stats[1] > v.smax) {
+        if (why) *why = "shift larger than the compiled pair range";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    if (p.n1 < (u64)(ns * (2 * v.smax + 6) + v.prefetch + kPairTaps + 8)) {
+        if (why) *why = "line shorter than the register ring";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    return BKMA_OK;
+}
+
 bkma_status stream_feasible(const bkma_adv_params &p, bkma_dtype dt, const StreamVariant &v, int ns,
                             std::string *why) {
-    if (v.dtype != dt) return BKMA_ERR_UNSUPPORTED;
+    if (v.dtype != dt || v.pair) return BKMA_ERR_UNSUPPORTED;
     if (ns < 1 || ns > v.nsmax) {
         if (why) *why = "steps per pass not compiled for this variant";
         return BKMA_ERR_UNSUPPORTED;
@@ -210,8 +311,9 @@ bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s, int ns) 
     grid = std::min<u64>(grid, 0x7fffffffULL / (u64)(64 * v.wpb));
     if (grid == 0) return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: empty grid");
     a.wave_stride = (i64)grid * v.wpb;
+    a.slab_dlo = pl.slab_dlo;
     v.fn(data, a, (unsigned)grid, s, ns);
-    return check_launch("adv_column_stream_kernel");
+    return check_launch(v.pair ? "adv_column_pair_kernel" : "adv_column_stream_kernel");
 }
 
 // nsteps steps: full passes of the plan's steps per pass, then the rest.
@@ -223,7 +325,7 @@ bkma_status plan_steps(const TunedPlan &pl, void *data, u64 nsteps, hipStream_t 
 }
 
 std::string stream_name(const StreamVariant &v, int ns, bool persistent) {
-    return std::string("column_stream<") + (v.dtype == BKMA_F64 ? "f64" : (v.wide ? "f32,win64" : "f32,win32")) +
+    return std::string(v.pair ? "column_pair<" : "column_stream<") + (v.dtype == BKMA_F64 ? "f64" : (v.wide ? "f32,win64" : "f32,win32")) +
            ",vec" + std::to_string(v.vec) + ",smax" + std::to_string(v.smax) + ",p" + std::to_string(v.prefetch) +
            ",wpb" + std::to_string(v.wpb) + ",steps" + std::to_string(ns) + (v.nt ? ",nt" : "") +
            (persistent ? ",persistent" : "") + ">";
@@ -255,13 +357,23 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
     pl->params = p;
     pl->dtype = dt;
     pl->cfg = cfg;
-    if (cfg.strategy == BKMA_STRATEGY_COLUMN_STREAM) {
+    if (cfg.strategy == BKMA_STRATEGY_COLUMN_STREAM || cfg.strategy == BKMA_STRATEGY_COLUMN_PAIR) {
         const int ns = steps_per_pass(cfg);
+        const int pair = cfg.strategy == BKMA_STRATEGY_COLUMN_PAIR;
         for (const auto &v : stream_variants(dt)) {
-            if (v.vec == cfg.vec && v.smax == cfg.smax && v.prefetch == cfg.prefetch && v.wpb == cfg.waves_per_wg &&
-                v.nt == (cfg.nontemporal != 0) && v.wide == (cfg.wide_window != 0) && ns <= v.nsmax) {
-                if (stream_feasible(p, dt, v, ns, &why) != BKMA_OK)
+            if (v.pair == pair && v.vec == cfg.vec && v.smax == cfg.smax && v.prefetch == cfg.prefetch &&
+                v.wpb == cfg.waves_per_wg && v.nt == (cfg.nontemporal != 0) && v.wide == (cfg.wide_window != 0) &&
+                ns <= v.nsmax) {
+                if (pair) {
+                    int stats[3];
+                    if (!shift_supported(p, &why))
+                        return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
+                    BKMA_TRY(scan_slab_shifts(p, &pl->slab_dlo, stats));
+                    if (pair_feasible(p, dt, v, ns, stats, &why) != BKMA_OK)
+                        return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
+                } else if (stream_feasible(p, dt, v, ns, &why) != BKMA_OK) {
                     return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
+                }
                 pl->variant = &v;
                 pl->ns = ns;
                 pl->cfg.steps_per_pass = ns;
@@ -299,17 +411,29 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
 // modes) and LINE_LDS with whole lines or segments sized for 32/64/160 KB.
 std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype dt, std::string *why) {
     std::vector<bkma_tuned_config> out;
+    // slab shift statistics for the column-pair variants (one device scan)
+    int stats[3] = {0, 0, 2};
+    bool have_stats = false;
+    for (const auto &v : stream_variants(dt))
+        if (v.pair && !have_stats && shift_supported(p, nullptr)) {
+            have_stats = scan_slab_shifts(p, nullptr, stats) == BKMA_OK;
+            if (!have_stats) stats[2] = 2;
+        }
+    auto feasible = [&](const StreamVariant &v, int ns) {
+        return v.pair ? pair_feasible(p, dt, v, ns, stats, why) == BKMA_OK
+                      : stream_feasible(p, dt, v, ns, why) == BKMA_OK;
+    };
     int smin_needed = 1 << 30;
     for (const auto &v : stream_variants(dt))
-        if (stream_feasible(p, dt, v, 1, why) == BKMA_OK) smin_needed = std::min(smin_needed, v.smax);
+        if (feasible(v, 1)) smin_needed = std::min(smin_needed, v.smax);
     for (const auto &v : stream_variants(dt)) {
         if (v.smax > smin_needed + 2) continue;
         for (int ns = 1; ns <= v.nsmax; ++ns) {
-            if (stream_feasible(p, dt, v, ns, why) != BKMA_OK) continue;
+            if (!feasible(v, ns)) continue;
             for (int pers = 0; pers < 2; ++pers) {
                 bkma_tuned_config c;
                 std::memset(&c, 0, sizeof(c));
-                c.strategy = BKMA_STRATEGY_COLUMN_STREAM;
+                c.strategy = v.pair ? BKMA_STRATEGY_COLUMN_PAIR : BKMA_STRATEGY_COLUMN_STREAM;
                 c.vec = v.vec;
                 c.smax = v.smax;
                 c.prefetch = v.prefetch;
@@ -351,12 +475,20 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
     return out;
 }
 
-// Heuristic (measured on the strided 2048x1024x512 workload): column stream
-// with the smallest window, FP32 at 2 and FP64 at 3 steps per pass (more
-// steps push the register window past the occupancy that hides HBM latency),
-// widest vector, deepest prefetch; else LINE_LDS whole line, else the
-// longest segment.
+// Heuristic (measured on the strided 2048x1024x512 workload): column pair
+// when every slab has at most two shifts (FP32 at 3, FP64 at 4 steps per
+// pass), else the column stream with the smallest window (FP32 at 2, FP64 at
+// 3 steps per pass; more steps push the register window past the occupancy
+// that hides HBM latency), widest vector, deepest prefetch; else LINE_LDS
+// whole line, else the longest segment.
 int heuristic_score(const bkma_tuned_config &c, bkma_dtype dt) {
+    if (c.strategy == BKMA_STRATEGY_COLUMN_PAIR) {
+        // FP32: FP64 stage-0 window, VEC 4, 3 steps; FP64: VEC 2, P 14, 4 steps
+        const int want = dt == BKMA_F64 ? 4 : 3;
+        const int ns = steps_per_pass(c);
+        return 1100000000 - c.smax * 1000000 - std::abs(ns - want) * 100000 + c.vec * 1000 + 500 * c.wide_window +
+               c.prefetch * 10 + c.nontemporal - c.persistent;
+    }
     if (c.strategy == BKMA_STRATEGY_COLUMN_STREAM) {
         const int want = dt == BKMA_F64 ? 3 : 2;
         const int ns = steps_per_pass(c);
@@ -497,7 +629,7 @@ extern "C" int32_t bkma_adv_plan_variants(bkma_dtype dtype, bkma_tuned_config *o
             if (out && n < max_out) {
                 bkma_tuned_config c;
                 std::memset(&c, 0, sizeof(c));
-                c.strategy = BKMA_STRATEGY_COLUMN_STREAM;
+                c.strategy = v.pair ? BKMA_STRATEGY_COLUMN_PAIR : BKMA_STRATEGY_COLUMN_STREAM;
                 c.vec = v.vec;
                 c.smax = v.smax;
                 c.prefetch = v.prefetch;

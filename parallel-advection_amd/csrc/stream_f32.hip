@@ -1,5 +1,8 @@
 // Compiled column-stream variants for FP32 storage (the tuner's search space).
-// Each entry instantiates steps-per-pass 1 .. NSMAX.
+// Each entry instantiates steps-per-pass 1 .. NSMAX.  BKMA_STREAM_VARIANT:
+// shift-ring kernel (adv_stream.hpp); BKMA_PAIR_VARIANT: per-slab shift
+// pair kernel (adv_pair.hpp).
+#include "adv_pair.hpp"
 #include "adv_stream.hpp"
 
 namespace bkma {
@@ -10,6 +13,7 @@ const StreamVariant *stream_variants_f32(int *count) {
     static const StreamVariant table[] = {
 #ifdef BKMA_TUNED_FEW
         BKMA_STREAM_VARIANT(float, float, BKMA_F32, 4, 2, 10, 1, true, 2),
+        BKMA_PAIR_VARIANT(float, double, BKMA_F32, 4, 2, 7, 1, true, 3),
 #else
         BKMA_STREAM_VARIANT(float, float, BKMA_F32, 4, 2, 10, 1, true, 3),
         BKMA_STREAM_VARIANT(float, float, BKMA_F32, 4, 2, 5, 1, true, 3),
@@ -21,6 +25,9 @@ const StreamVariant *stream_variants_f32(int *count) {
         BKMA_STREAM_VARIANT(float, float, BKMA_F32, 4, 4, 7, 1, true, 2),
         BKMA_STREAM_VARIANT(float, float, BKMA_F32, 2, 4, 14, 1, true, 3),
         BKMA_STREAM_VARIANT(float, float, BKMA_F32, 1, 4, 14, 1, false, 3),
+        BKMA_PAIR_VARIANT(float, float, BKMA_F32, 4, 2, 7, 1, true, 4),
+        BKMA_PAIR_VARIANT(float, double, BKMA_F32, 4, 2, 7, 1, true, 3),
+        BKMA_PAIR_VARIANT(float, float, BKMA_F32, 2, 2, 14, 1, true, 5),
 #endif
     };
     *count = (int)(sizeof(table) / sizeof(table[0]));

@@ -100,6 +100,45 @@ def test_every_f64_variant(cuda, cfg):
     assert np.array_equal(got, ref)
 
 
+PAIR_VARIANTS = [(B.F32, c) for c in F32_VARIANTS if c.strategy == B.STRATEGY_COLUMN_PAIR] + \
+    [(B.F64, c) for c in F64_VARIANTS if c.strategy == B.STRATEGY_COLUMN_PAIR]
+
+
+@pytest.mark.parametrize("dtype,cfg", PAIR_VARIANTS,
+                         ids=[("f32-" if d == B.F32 else "f64-") + _variant_id(c) for d, c in PAIR_VARIANTS])
+def test_pair_every_shift_pair(cuda, dtype, cfg):
+    """Column-pair kernels: slabs with shifts s = v*dt/dx from -2.8 to 1.68
+    (rows shifted by d = ceil(s) in {-2, -1, 0, 1, 2}, a zero-velocity slab)
+    reach every compiled pair instance; results are bit-identical to the
+    oracle."""
+    n0, n1 = 9, 300
+    n2 = 64 * cfg.vec
+    dt = 2.8 / n1  # dx = 1/n1, v in {-1, -0.8, ..., 0.6}
+    p = B.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=0.8)
+    po = O.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=0.8)
+    if dtype == B.F32:
+        host = O.uniform(n0 * n1 * n2, 5, np.float32).reshape(n0, n1, n2)
+    else:
+        host = O.uniform(n0 * n1 * n2, 6).reshape(n0, n1, n2)
+    nsteps = 2 * cfg.steps_per_pass + 1
+    got, spp = _run(host, p, dtype, nsteps, config=cfg)
+    assert spp == cfg.steps_per_pass
+    ref = host.copy()
+    if dtype == B.F32:
+        O.advect_f32_blocked(ref, po, nsteps, spp)
+    else:
+        O.advect_f64(ref, po, nsteps)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+
+
+def test_pair_shift_beyond_range_rejected(cuda):
+    cfg = next(c for c in F32_VARIANTS if c.strategy == B.STRATEGY_COLUMN_PAIR)
+    n1 = 300
+    p = B.make_params(2, n1, 256, dt=2.5 / n1, minRealVx=0, maxRealVx=2)  # s = 2.5: d = 3
+    with pytest.raises(B.BkmaUnsupported):
+        B.AdvPlan(p, B.F32, config=cfg)
+
+
 def test_integer_shift_known_answer(cuda):
     # v*dt/dx integer (dt = 1/512, n1 = 1024, v = -1 + i0/2): circular shift
     n0, n1, n2 = 4, 1024, 128
