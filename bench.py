@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "adv_column_stream_f32_2048x1024x512.traffic.txt"),
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "adv_column_pair_f32_2048x1024x512.traffic.txt"),
                     help="JSON summary of the rocprofv3 FETCH_SIZE/WRITE_SIZE passes (hbm_bytes_per_launch)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (3 launches), for scaling rocprofv3 byte counters")

@@ -200,7 +200,8 @@ typedef struct bkma_tuned_config {
     int32_t wide_window;  /* COLUMN_STREAM, F32 storage: 1 = FP64 register window (rows widened once) */
     int32_t steps_per_pass; /* COLUMN_STREAM: time steps fused into one pass over HBM (0 or 1: one) */
     int32_t chunk_segs;   /* LINE_LDS segmented: consecutive segments one work-group walks in order (0 or 1: one) */
-    int32_t _reserved;
+    int32_t dpp_broadcast; /* COLUMN_PAIR: 1 = the coefficients reach the stencil sums through the DPP row
+                              broadcast; 0 = through v_readlane */
 } bkma_tuned_config;
 
 typedef struct bkma_adv_plan bkma_adv_plan;

@@ -100,7 +100,7 @@ class TunedConfig(ctypes.Structure):
                 ("prefetch", ctypes.c_int32), ("waves_per_wg", ctypes.c_int32),
                 ("nontemporal", ctypes.c_int32), ("persistent", ctypes.c_int32), ("seg_rows", ctypes.c_int32),
                 ("wide_window", ctypes.c_int32), ("steps_per_pass", ctypes.c_int32),
-                ("chunk_segs", ctypes.c_int32), ("_reserved", ctypes.c_int32)]
+                ("chunk_segs", ctypes.c_int32), ("dpp_broadcast", ctypes.c_int32)]
 
 
 STRATEGY_COLUMN_STREAM = 1

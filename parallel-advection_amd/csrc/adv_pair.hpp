@@ -20,6 +20,17 @@
 // the stage lag (3 - D rows) and the window (seven rows per stage) at
 // compile time.  Periodic wrap, in-place order, the head rows kept in LDS,
 // the foot table and the FP64 intermediate stages follow adv_stream.hpp.
+//
+// Coefficient broadcast.  Each loop trip evaluates the foot and lag_basis
+// of its rows, one row per lane.  DPP = false: every stage reads its row's
+// seven coefficients back with v_readlane (14 VALU instructions per row and
+// stage).  DPP = true (FP32 storage): the trip's table is copied into sets
+// of 16 rows, each replicated in the four 16-lane rows of the wavefront
+// (ds_bpermute), and the fused multiply-adds read the coefficient of table
+// lane b straight from that set with the DPP modifier row_newbcast:b, so the
+// broadcast costs no instruction of its own.  (For FP64 storage, whose
+// separately rounded multiply has no DPP form, a 64-bit DPP move per
+// coefficient measured no faster than v_readlane.)
 #pragma once
 
 #include "adv_stream.hpp"
@@ -60,13 +71,61 @@ __device__ __forceinline__ void pair_taps(const WT (&win)[kPairTaps][VEC], const
     }
 }
 
+// The fused sum (FP32 storage) with the coefficients taken from lane B of
+// each 16-lane row of `set`: v_fmac_f64 with the DPP source modifier
+// row_newbcast:B.  The leading s_nop 1 covers the two wait states a DPP read
+// needs after a VALU write of its source register.
+template <class T, class WT, class OT, int VEC, int S0, int B>
+__device__ __forceinline__ void pair_taps_dpp(const WT (&win)[kPairTaps][VEC], const double (&set)[kPairTaps],
+                                              OT (&out)[VEC]) {
+    static_assert(B >= 0 && B < 16, "row_newbcast lane");
+    static_assert(sizeof(T) == 4, "DPP broadcast: fused (FP32 storage) sums only");
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+        double w[kPairTaps];
+#pragma unroll
+        for (int k = 0; k < kPairTaps; ++k) w[k] = (double)win[(S0 + k) % kPairTaps][v];
+        double acc = 0.;
+        asm("s_nop 1\n\t"
+            "v_fmac_f64_dpp %0, %1, %8 row_newbcast:%c15 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %0, %2, %9 row_newbcast:%c15 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %0, %3, %10 row_newbcast:%c15 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %0, %4, %11 row_newbcast:%c15 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %0, %5, %12 row_newbcast:%c15 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %0, %6, %13 row_newbcast:%c15 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %0, %7, %14 row_newbcast:%c15 row_mask:0xf bank_mask:0xf"
+            : "+v"(acc)
+            : "v"(set[0]), "v"(set[1]), "v"(set[2]), "v"(set[3]), "v"(set[4]), "v"(set[5]), "v"(set[6]),
+              "v"(w[0]), "v"(w[1]), "v"(w[2]), "v"(w[3]), "v"(w[4]), "v"(w[5]), "v"(w[6]), "n"(B));
+        out[v] = (OT)acc;
+    }
+}
+
+// Value of `x` in lane `src` of the wavefront (ds_bpermute, 64-bit).
+__device__ __forceinline__ double lane_gather(double x, int src) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_ds_bpermute(src * 4, (int)(unsigned)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_ds_bpermute(src * 4, (int)(unsigned)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// The seven coefficients of a row with shift d, for the pair {D, D+1}.
+template <int D> __device__ __forceinline__ void pair_coefs(int d, const double (&c)[6], double (&out)[kPairTaps]) {
+    const bool late = d == D;  // taps start one row later in the window
+    out[0] = late ? 0. : c[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k) out[k] = late ? c[k - 1] : c[k];
+    out[6] = late ? c[5] : 0.;
+}
+
 // One wavefront advances column tile `tile` of slab i0 by NS steps; every
 // row's shift is D or D+1.
-template <class T, class WT, int VEC, int P, int NS, bool NT, int D>
+template <class T, class WT, int VEC, int P, int NS, bool NT, bool DPP, int D>
 __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs &a, i64 i0, i64 tile, int lane,
                                           T (*head)[64 * VEC]) {
     using S = PairShape<D, P, NS>;
     constexpr int AHEAD = S::AHEAD, BACK = S::BACK, W = S::W, U = S::U, R = S::R, H = S::H, TB = S::TB;
+    constexpr int NSET = DPP ? (U + TB - AHEAD + 15) / 16 : 1;  // 16-row sets of the table the stages read
     constexpr int NM = NS > 1 ? NS - 1 : 1;
     using M = VecMem<T, VEC, NT>;
     const AdvDev &p = a.p;
@@ -105,6 +164,7 @@ __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs
 
     const int half = (int)(n1 / 2);
     double tab[kPairTaps];
+    double tset[NSET][kPairTaps];
 
     for (i64 base = 0; base < total; base += U) {
         // Foot table: lane l holds the seven coefficients of the row at
@@ -120,11 +180,14 @@ __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs
             int d = (int)row - leftNode;
             if (d > half) d -= (int)n1;
             if (d < -half) d += (int)n1;
-            const bool late = d == D;  // taps start one row later in the window
-            tab[0] = late ? 0. : c[0];
+            pair_coefs<D>(d, c, tab);
+        }
+        if constexpr (DPP) {
+            // set s, lane l: table entry 16*s + l%16
 #pragma unroll
-            for (int k = 1; k < 6; ++k) tab[k] = late ? c[k - 1] : c[k];
-            tab[6] = late ? c[5] : 0.;
+            for (int s = 0; s < NSET; ++s)
+#pragma unroll
+                for (int k = 0; k < kPairTaps; ++k) tset[s][k] = lane_gather(tab[k], 16 * s + (lane & 15));
         }
 
         auto step = [&](auto jc) __attribute__((always_inline)) {
@@ -157,22 +220,32 @@ __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs
                 constexpr int L = OFF + TB;
                 constexpr int S0 = cmod(OFF - BACK, W);
                 double c[kPairTaps];
+                if constexpr (!DPP) {
 #pragma unroll
-                for (int k = 0; k < kPairTaps; ++k) c[k] = lane_value<L>(tab[k]);
+                    for (int k = 0; k < kPairTaps; ++k) c[k] = lane_value<L>(tab[k]);
+                }
+                auto taps = [&](const auto &win, auto &out) __attribute__((always_inline)) {
+                    using WTT = std::remove_cv_t<std::remove_reference_t<decltype(win[0][0])>>;
+                    using OTT = std::remove_reference_t<decltype(out[0])>;
+                    if constexpr (DPP)
+                        pair_taps_dpp<T, WTT, OTT, VEC, S0, L % 16>(win, tset[L / 16], out);
+                    else
+                        pair_taps<T, WTT, OTT, VEC, S0>(win, c, out);
+                };
                 if constexpr (t < NS) {
                     double o[VEC];
                     if constexpr (t == 1)
-                        pair_taps<T, WT, double, VEC, S0>(w0, c, o);
+                        taps(w0, o);
                     else
-                        pair_taps<T, double, double, VEC, S0>(wm[t - 2], c, o);
+                        taps(wm[t - 2], o);
 #pragma unroll
                     for (int v = 0; v < VEC; ++v) wm[t - 1][cmod(OFF, W)][v] = o[v];
                 } else {
                     T o[VEC];
                     if constexpr (NS == 1)
-                        pair_taps<T, WT, T, VEC, S0>(w0, c, o);
+                        taps(w0, o);
                     else
-                        pair_taps<T, double, T, VEC, S0>(wm[NS - 2], c, o);
+                        taps(wm[NS - 2], o);
                     const i64 pt = pin - NS * AHEAD;
                     if (pt >= R && pt < first_end && lane_ok) M::store(slab + (pt - R) * n2, o);
                 }
@@ -183,30 +256,30 @@ __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs
 }
 
 // Selects the instance of D (wave-uniform) in [DLO, DHI].
-template <class T, class WT, int VEC, int P, int NS, bool NT, int DLO, int DHI>
+template <class T, class WT, int VEC, int P, int NS, bool NT, bool DPP, int DLO, int DHI>
 __device__ __forceinline__ void pair_dispatch(int d, T *__restrict__ data, const StreamArgs &a, i64 i0, i64 tile,
                                               int lane, T (*head)[64 * VEC]) {
     if constexpr (DLO <= DHI) {
         if (d == DLO) {
-            pair_unit<T, WT, VEC, P, NS, NT, DLO>(data, a, i0, tile, lane, head);
+            pair_unit<T, WT, VEC, P, NS, NT, DPP, DLO>(data, a, i0, tile, lane, head);
             return;
         }
-        pair_dispatch<T, WT, VEC, P, NS, NT, DLO + 1, DHI>(d, data, a, i0, tile, lane, head);
+        pair_dispatch<T, WT, VEC, P, NS, NT, DPP, DLO + 1, DHI>(d, data, a, i0, tile, lane, head);
     }
 }
 
 // Slabs with shifts in [-SMAX, SMAX]: D in [-SMAX, SMAX-1] (a slab whose
 // only shift is SMAX runs as the pair {SMAX-1, SMAX}).
-template <class T, class WT, int VEC, int SMAX, int P, int NS, int WPB, bool NT>
+template <class T, class WT, int VEC, int SMAX, int P, int NS, int WPB, bool NT, bool DPP>
 __global__ __launch_bounds__(64 * WPB) void adv_column_pair_kernel(T *__restrict__ data, StreamArgs a) {
     __shared__ T head_lds[WPB][NS * (SMAX + 3)][64 * VEC];
     const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
+    const int wv = wave_uniform((int)(threadIdx.x >> 6));  // unit indices and addresses stay scalar
     for (i64 unit = (i64)blockIdx.x * WPB + wv; unit < a.total_units; unit += a.wave_stride) {
         const i64 i0 = unit / a.tiles_per_slab;
         const i64 tile = unit - i0 * a.tiles_per_slab;
         const int d = wave_uniform(min(a.slab_dlo[i0], SMAX - 1));
-        pair_dispatch<T, WT, VEC, P, NS, NT, -SMAX, SMAX - 1>(d, data, a, i0, tile, lane, head_lds[wv]);
+        pair_dispatch<T, WT, VEC, P, NS, NT, DPP, -SMAX, SMAX - 1>(d, data, a, i0, tile, lane, head_lds[wv]);
     }
 }
 
@@ -215,39 +288,41 @@ __global__ __launch_bounds__(64 * WPB) void adv_column_pair_kernel(T *__restrict
 // max over slabs of (max d - min d)} (stats preset to {INT_MAX, INT_MIN, 0}).
 __global__ void slab_shift_scan_kernel(AdvDev p, int *__restrict__ dlo, int *__restrict__ stats);
 
+
 // ---------------------------------------------------------------------------
 // Host side: launchers for steps-per-pass 1 .. NSMAX of one variant
 // ---------------------------------------------------------------------------
-template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, int NS>
+template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, bool DPP, int NS>
 void launch_pair(void *data, const StreamArgs &a, unsigned grid, hipStream_t s, int ns) {
     if constexpr (NS >= 1) {
         if (ns == NS) {
-            hipLaunchKernelGGL((adv_column_pair_kernel<T, WT, VEC, SMAX, P, NS, WPB, NT>), dim3(grid), dim3(64 * WPB),
-                               0, s, reinterpret_cast<T *>(data), a);
+            hipLaunchKernelGGL((adv_column_pair_kernel<T, WT, VEC, SMAX, P, NS, WPB, NT, DPP>), dim3(grid),
+                               dim3(64 * WPB), 0, s, reinterpret_cast<T *>(data), a);
             return;
         }
-        launch_pair<T, WT, VEC, SMAX, P, WPB, NT, NS - 1>(data, a, grid, s, ns);
+        launch_pair<T, WT, VEC, SMAX, P, WPB, NT, DPP, NS - 1>(data, a, grid, s, ns);
     }
 }
 
-template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, int NS> int occupancy_pair(int ns) {
+template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, bool DPP, int NS> int occupancy_pair(int ns) {
     if constexpr (NS >= 1) {
         if (ns == NS) {
             int blocks = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &blocks, adv_column_pair_kernel<T, WT, VEC, SMAX, P, NS, WPB, NT>, 64 * WPB, 0) != hipSuccess)
+                    &blocks, adv_column_pair_kernel<T, WT, VEC, SMAX, P, NS, WPB, NT, DPP>, 64 * WPB, 0) !=
+                hipSuccess)
                 return 0;
             return blocks;
         }
-        return occupancy_pair<T, WT, VEC, SMAX, P, WPB, NT, NS - 1>(ns);
+        return occupancy_pair<T, WT, VEC, SMAX, P, WPB, NT, DPP, NS - 1>(ns);
     }
     return 0;
 }
 
-#define BKMA_PAIR_VARIANT(T, WT, DT, V, S, P, W, NT, NSMAX)                                                   \
-    StreamVariant {                                                                                          \
-        DT, 1, V, S, P, W, NT, sizeof(WT) > sizeof(T), NSMAX, &launch_pair<T, WT, V, S, P, W, NT, NSMAX>,    \
-            &occupancy_pair<T, WT, V, S, P, W, NT, NSMAX>                                                    \
+#define BKMA_PAIR_VARIANT(T, WT, DT, V, S, P, W, NT, DPP, NSMAX)                                                \
+    StreamVariant {                                                                                            \
+        DT, 1, V, S, P, W, NT, sizeof(WT) > sizeof(T), NSMAX, &launch_pair<T, WT, V, S, P, W, NT, DPP, NSMAX>, \
+            &occupancy_pair<T, WT, V, S, P, W, NT, DPP, NSMAX>, DPP                                            \
     }
 
 }  // namespace bkma

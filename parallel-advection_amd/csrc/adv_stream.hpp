@@ -268,7 +268,7 @@ __global__ __launch_bounds__(64 * WPB) void adv_column_stream_kernel(T *__restri
     using S = StreamShape<SMAX, P, NS>;
     __shared__ T head_lds[WPB][S::H][64 * VEC];
     const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
+    const int wv = wave_uniform((int)(threadIdx.x >> 6));  // unit indices and addresses stay scalar
     // Persistent waves: unit = wave, wave + stride, ...  (all lanes of a wave
     // take the same trip count; there is no work-group barrier).
     for (i64 unit = (i64)blockIdx.x * WPB + wv; unit < a.total_units; unit += a.wave_stride)
@@ -315,12 +315,13 @@ struct StreamVariant {
     int nsmax;                               // steps per pass compiled: 1 .. nsmax
     StreamLaunchFn fn;
     StreamOccFn occ;
+    int dpp;  // column pair: coefficients broadcast by DPP (1) or v_readlane (0)
 };
 
 #define BKMA_STREAM_VARIANT(T, WT, DT, V, S, P, W, NT, NSMAX)                                                  \
     StreamVariant {                                                                                           \
         DT, 0, V, S, P, W, NT, sizeof(WT) > sizeof(T), NSMAX, &launch_stream<T, WT, V, S, P, W, NT, NSMAX>,   \
-            &occupancy_stream<T, WT, V, S, P, W, NT, NSMAX>                                                   \
+            &occupancy_stream<T, WT, V, S, P, W, NT, NSMAX>, 0                                                \
     }
 
 // The compiled variants of each storage type (host functions), both kernels.

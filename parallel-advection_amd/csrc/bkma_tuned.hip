@@ -328,7 +328,7 @@ std::string stream_name(const StreamVariant &v, int ns, bool persistent) {
     return std::string(v.pair ? "column_pair<" : "column_stream<") + (v.dtype == BKMA_F64 ? "f64" : (v.wide ? "f32,win64" : "f32,win32")) +
            ",vec" + std::to_string(v.vec) + ",smax" + std::to_string(v.smax) + ",p" + std::to_string(v.prefetch) +
            ",wpb" + std::to_string(v.wpb) + ",steps" + std::to_string(ns) + (v.nt ? ",nt" : "") +
-           (persistent ? ",persistent" : "") + ">";
+           (v.dpp ? ",dpp" : "") + (persistent ? ",persistent" : "") + ">";
 }
 
 std::string lds_name(bkma_dtype dt, const LdsArgs &a) {
@@ -363,7 +363,7 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
         for (const auto &v : stream_variants(dt)) {
             if (v.pair == pair && v.vec == cfg.vec && v.smax == cfg.smax && v.prefetch == cfg.prefetch &&
                 v.wpb == cfg.waves_per_wg && v.nt == (cfg.nontemporal != 0) && v.wide == (cfg.wide_window != 0) &&
-                ns <= v.nsmax) {
+                v.dpp == (cfg.dpp_broadcast != 0) && ns <= v.nsmax) {
                 if (pair) {
                     int stats[3];
                     if (!shift_supported(p, &why))
@@ -440,6 +440,7 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
                 c.waves_per_wg = v.wpb;
                 c.nontemporal = v.nt;
                 c.wide_window = v.wide;
+                c.dpp_broadcast = v.dpp;
                 c.steps_per_pass = ns;
                 c.persistent = pers;
                 out.push_back(c);
@@ -476,18 +477,20 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
 }
 
 // Heuristic (measured on the strided 2048x1024x512 workload): column pair
-// when every slab has at most two shifts (FP32 at 3, FP64 at 4 steps per
-// pass), else the column stream with the smallest window (FP32 at 2, FP64 at
+// when every slab has at most two shifts (FP32 with the DPP broadcast at 5,
+// FP64 at 4 steps per pass), else the column stream with the smallest window (FP32 at 2, FP64 at
 // 3 steps per pass; more steps push the register window past the occupancy
 // that hides HBM latency), widest vector, deepest prefetch; else LINE_LDS
 // whole line, else the longest segment.
 int heuristic_score(const bkma_tuned_config &c, bkma_dtype dt) {
     if (c.strategy == BKMA_STRATEGY_COLUMN_PAIR) {
-        // FP32: FP64 stage-0 window, VEC 4, 3 steps; FP64: VEC 2, P 14, 4 steps
-        const int want = dt == BKMA_F64 ? 4 : 3;
+        // FP32: DPP broadcast, FP64 stage-0 window, VEC 2, P 14, 5 steps;
+        // FP64: VEC 2, P 14, 4 steps
+        const int want = dt == BKMA_F64 ? 4 : (c.dpp_broadcast ? 5 : 3);
         const int ns = steps_per_pass(c);
-        return 1100000000 - c.smax * 1000000 - std::abs(ns - want) * 100000 + c.vec * 1000 + 500 * c.wide_window +
-               c.prefetch * 10 + c.nontemporal - c.persistent;
+        return 1100000000 + c.dpp_broadcast * 10000000 - c.smax * 1000000 - std::abs(ns - want) * 100000 +
+               500 * c.wide_window + c.prefetch * 10 + c.nontemporal - c.persistent +
+               (c.dpp_broadcast ? -c.vec : c.vec) * 1000;
     }
     if (c.strategy == BKMA_STRATEGY_COLUMN_STREAM) {
         const int want = dt == BKMA_F64 ? 3 : 2;
@@ -636,6 +639,7 @@ extern "C" int32_t bkma_adv_plan_variants(bkma_dtype dtype, bkma_tuned_config *o
                 c.waves_per_wg = v.wpb;
                 c.nontemporal = v.nt;
                 c.wide_window = v.wide;
+                c.dpp_broadcast = v.dpp;
                 c.steps_per_pass = ns;
                 out[n] = c;
             }

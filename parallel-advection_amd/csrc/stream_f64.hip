@@ -7,12 +7,12 @@
 
 namespace bkma {
 
-// (storage, stage-0 window, dtype, VEC, SMAX, P, waves/WG, non-temporal, NSMAX)
+// (storage, stage-0 window, dtype, VEC, SMAX, P, waves/WG, non-temporal, [pair: DPP broadcast (FP32 only),] NSMAX)
 const StreamVariant *stream_variants_f64(int *count) {
     static const StreamVariant table[] = {
 #ifdef BKMA_TUNED_FEW
         BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 2, 5, 1, true, 2),
-        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 2, 14, 1, true, 4),
+        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 2, 14, 1, true, false, 4),
 #else
         BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 2, 5, 1, true, 3),
         BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 2, 10, 1, true, 3),
@@ -20,8 +20,8 @@ const StreamVariant *stream_variants_f64(int *count) {
         BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 1, 8, 1, true, 3),
         BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 4, 7, 1, true, 2),
         BKMA_STREAM_VARIANT(double, double, BKMA_F64, 1, 4, 14, 1, false, 3),
-        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 2, 7, 1, true, 6),
-        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 2, 14, 1, true, 5),
+        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 2, 7, 1, true, false, 6),
+        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 2, 14, 1, true, false, 5),
 #endif
     };
     *count = (int)(sizeof(table) / sizeof(table[0]));
