@@ -7,7 +7,10 @@
 // below one cell: d = ceil(s - e) is the same for every row unless s is
 // within e of an integer, and then it takes two adjacent values.  A plan
 // scans every (i0, i1) once with the device's own foot arithmetic
-// (slab_shift_scan_kernel) and keeps D = min d of each slab.
+// (slab_shift_scan_kernel) and keeps the smallest and largest d of each
+// slab.  A slab with one shift D (almost every slab) runs the reference's
+// six taps over rows i1-D-2 .. i1-D+3 with six-row windows; a slab with two
+// shifts runs the seven-tap form below.
 //
 // The six taps of a row with shift d read rows i1-d-2 .. i1-d+3.  Over the
 // seven rows i1-D-3 .. i1-D+3 they are the seven coefficients
@@ -39,33 +42,36 @@ namespace bkma {
 
 constexpr int kPairTaps = 7;
 
-// Stage windows for a slab whose shifts are {D, D+1}.
-template <int D, int P, int NS> struct PairShape {
-    static constexpr int BACK = D + 3;   // rows behind the output row read by its taps
-    static constexpr int AHEAD = 3 - D;  // rows in front of it (stage lag)
-    static constexpr int W = kPairTaps;  // rows per stage window
+// Stage windows of K taps (K = 6: one shift D; K = 7: shifts {D, D+1}).
+// PV is the variant's prefetch depth; the six-tap instance rounds it down to
+// a multiple of six so that one loop trip stays one window long.
+template <int K, int D, int PV, int NS> struct PairShape {
+    static constexpr int BACK = D + K - 4;  // rows behind the output row read by its taps
+    static constexpr int AHEAD = 3 - D;     // rows in front of it (stage lag)
+    static constexpr int W = K;             // rows per stage window
+    static constexpr int P = K == 7 ? PV : (PV >= 12 ? 12 : 6);
     static constexpr int U = clcm(W, P);
     static constexpr int R = NS * BACK;
     static constexpr int H = NS * AHEAD;
     static constexpr int TB = NS * AHEAD;
-    static_assert(BACK >= 0 && AHEAD >= 1, "shift pair outside the supported range");
+    static_assert(K == 6 || K == 7, "six or seven taps");
+    static_assert(BACK >= 0 && AHEAD >= 1, "shift outside the supported range");
     static_assert(U + TB <= 64, "foot table must fit one wavefront");
 };
 
-// Seven-tap sum of VEC cells from window slots S0..S0+6 (mod W), in FP64:
+// K-tap sum of VEC cells from window slots S0..S0+K-1 (mod K), in FP64:
 // separately rounded multiply and add for FP64 storage, fused for FP32.
-template <class T, class WT, class OT, int VEC, int S0>
-__device__ __forceinline__ void pair_taps(const WT (&win)[kPairTaps][VEC], const double (&c)[kPairTaps],
-                                          OT (&out)[VEC]) {
+template <class T, class WT, class OT, int K, int VEC, int S0>
+__device__ __forceinline__ void pair_taps(const WT (&win)[K][VEC], const double (&c)[kPairTaps], OT (&out)[VEC]) {
 #pragma unroll
     for (int v = 0; v < VEC; ++v) {
         double acc = 0.;
         if constexpr (sizeof(T) == 8) {
 #pragma unroll
-            for (int k = 0; k < kPairTaps; ++k) acc = acc + c[k] * (double)win[(S0 + k) % kPairTaps][v];
+            for (int k = 0; k < K; ++k) acc = acc + c[k] * (double)win[(S0 + k) % K][v];
         } else {
 #pragma unroll
-            for (int k = 0; k < kPairTaps; ++k) acc = fma(c[k], (double)win[(S0 + k) % kPairTaps][v], acc);
+            for (int k = 0; k < K; ++k) acc = fma(c[k], (double)win[(S0 + k) % K][v], acc);
         }
         out[v] = (OT)acc;
     }
@@ -75,8 +81,8 @@ __device__ __forceinline__ void pair_taps(const WT (&win)[kPairTaps][VEC], const
 // each 16-lane row of `set`: v_fmac_f64 with the DPP source modifier
 // row_newbcast:B.  The leading s_nop 1 covers the two wait states a DPP read
 // needs after a VALU write of its source register.
-template <class T, class WT, class OT, int VEC, int S0, int B>
-__device__ __forceinline__ void pair_taps_dpp(const WT (&win)[kPairTaps][VEC], const double (&set)[kPairTaps],
+template <class T, class WT, class OT, int K, int VEC, int S0, int B>
+__device__ __forceinline__ void pair_taps_dpp(const WT (&win)[K][VEC], const double (&set)[kPairTaps],
                                               OT (&out)[VEC]) {
     static_assert(B >= 0 && B < 16, "row_newbcast lane");
     static_assert(sizeof(T) == 4, "DPP broadcast: fused (FP32 storage) sums only");
@@ -84,8 +90,22 @@ __device__ __forceinline__ void pair_taps_dpp(const WT (&win)[kPairTaps][VEC], c
     for (int v = 0; v < VEC; ++v) {
         double w[kPairTaps];
 #pragma unroll
-        for (int k = 0; k < kPairTaps; ++k) w[k] = (double)win[(S0 + k) % kPairTaps][v];
+        for (int k = 0; k < K; ++k) w[k] = (double)win[(S0 + k) % K][v];
         double acc = 0.;
+        if constexpr (K == 6) {
+            asm("s_nop 1\n\t"
+                "v_fmac_f64_dpp %0, %1, %7 row_newbcast:%c13 row_mask:0xf bank_mask:0xf\n\t"
+                "v_fmac_f64_dpp %0, %2, %8 row_newbcast:%c13 row_mask:0xf bank_mask:0xf\n\t"
+                "v_fmac_f64_dpp %0, %3, %9 row_newbcast:%c13 row_mask:0xf bank_mask:0xf\n\t"
+                "v_fmac_f64_dpp %0, %4, %10 row_newbcast:%c13 row_mask:0xf bank_mask:0xf\n\t"
+                "v_fmac_f64_dpp %0, %5, %11 row_newbcast:%c13 row_mask:0xf bank_mask:0xf\n\t"
+                "v_fmac_f64_dpp %0, %6, %12 row_newbcast:%c13 row_mask:0xf bank_mask:0xf"
+                : "+v"(acc)
+                : "v"(set[0]), "v"(set[1]), "v"(set[2]), "v"(set[3]), "v"(set[4]), "v"(set[5]), "v"(w[0]),
+                  "v"(w[1]), "v"(w[2]), "v"(w[3]), "v"(w[4]), "v"(w[5]), "n"(B));
+            out[v] = (OT)acc;
+            continue;
+        }
         asm("s_nop 1\n\t"
             "v_fmac_f64_dpp %0, %1, %8 row_newbcast:%c15 row_mask:0xf bank_mask:0xf\n\t"
             "v_fmac_f64_dpp %0, %2, %9 row_newbcast:%c15 row_mask:0xf bank_mask:0xf\n\t"
@@ -119,12 +139,12 @@ template <int D> __device__ __forceinline__ void pair_coefs(int d, const double 
 }
 
 // One wavefront advances column tile `tile` of slab i0 by NS steps; every
-// row's shift is D or D+1.
-template <class T, class WT, int VEC, int P, int NS, bool NT, bool DPP, int D>
+// row's shift is D (K = 6) or D or D+1 (K = 7).
+template <class T, class WT, int VEC, int PV, int NS, bool NT, bool DPP, int K, int D>
 __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs &a, i64 i0, i64 tile, int lane,
                                           T (*head)[64 * VEC]) {
-    using S = PairShape<D, P, NS>;
-    constexpr int AHEAD = S::AHEAD, BACK = S::BACK, W = S::W, U = S::U, R = S::R, H = S::H, TB = S::TB;
+    using S = PairShape<K, D, PV, NS>;
+    constexpr int AHEAD = S::AHEAD, BACK = S::BACK, W = S::W, P = S::P, U = S::U, R = S::R, H = S::H, TB = S::TB;
     constexpr int NSET = DPP ? (U + TB - AHEAD + 15) / 16 : 1;  // 16-row sets of the table the stages read
     constexpr int NM = NS > 1 ? NS - 1 : 1;
     using M = VecMem<T, VEC, NT>;
@@ -167,8 +187,8 @@ __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs
     double tset[NSET][kPairTaps];
 
     for (i64 base = 0; base < total; base += U) {
-        // Foot table: lane l holds the seven coefficients of the row at
-        // position base - TB + l.
+        // Foot table: lane l holds the K coefficients of the row at position
+        // base - TB + l.
         {
             i64 row = base - TB + lane - R;
             if (row < 0) row += n1;
@@ -177,17 +197,22 @@ __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs
             double d_prev1, c[6];
             const int leftNode = adv_foot(p, (int)i0, (int)row, d_prev1);
             adv_lag_basis(d_prev1, c);
-            int d = (int)row - leftNode;
-            if (d > half) d -= (int)n1;
-            if (d < -half) d += (int)n1;
-            pair_coefs<D>(d, c, tab);
+            if constexpr (K == 7) {
+                int d = (int)row - leftNode;
+                if (d > half) d -= (int)n1;
+                if (d < -half) d += (int)n1;
+                pair_coefs<D>(d, c, tab);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) tab[k] = c[k];
+            }
         }
         if constexpr (DPP) {
             // set s, lane l: table entry 16*s + l%16
 #pragma unroll
             for (int s = 0; s < NSET; ++s)
 #pragma unroll
-                for (int k = 0; k < kPairTaps; ++k) tset[s][k] = lane_gather(tab[k], 16 * s + (lane & 15));
+                for (int k = 0; k < K; ++k) tset[s][k] = lane_gather(tab[k], 16 * s + (lane & 15));
         }
 
         auto step = [&](auto jc) __attribute__((always_inline)) {
@@ -212,7 +237,7 @@ __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs
                     for (int v = 0; v < VEC; ++v) w0[sw][v] = (WT)head[h][lane * VEC + v];
                 }
             }
-            // 3. stage t computes position pin - t*AHEAD from the seven rows
+            // 3. stage t computes position pin - t*AHEAD from the K rows
             //    around it in stage t-1
             static_for<1, NS + 1>([&](auto tc) __attribute__((always_inline)) {
                 constexpr int t = decltype(tc)::value;
@@ -222,15 +247,15 @@ __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs
                 double c[kPairTaps];
                 if constexpr (!DPP) {
 #pragma unroll
-                    for (int k = 0; k < kPairTaps; ++k) c[k] = lane_value<L>(tab[k]);
+                    for (int k = 0; k < K; ++k) c[k] = lane_value<L>(tab[k]);
                 }
                 auto taps = [&](const auto &win, auto &out) __attribute__((always_inline)) {
                     using WTT = std::remove_cv_t<std::remove_reference_t<decltype(win[0][0])>>;
                     using OTT = std::remove_reference_t<decltype(out[0])>;
                     if constexpr (DPP)
-                        pair_taps_dpp<T, WTT, OTT, VEC, S0, L % 16>(win, tset[L / 16], out);
+                        pair_taps_dpp<T, WTT, OTT, K, VEC, S0, L % 16>(win, tset[L / 16], out);
                     else
-                        pair_taps<T, WTT, OTT, VEC, S0>(win, c, out);
+                        pair_taps<T, WTT, OTT, K, VEC, S0>(win, c, out);
                 };
                 if constexpr (t < NS) {
                     double o[VEC];
@@ -255,38 +280,46 @@ __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs
     }
 }
 
-// Selects the instance of D (wave-uniform) in [DLO, DHI].
-template <class T, class WT, int VEC, int P, int NS, bool NT, bool DPP, int DLO, int DHI>
+// Selects the K-tap instance of D (wave-uniform) in [DLO, DHI].
+template <class T, class WT, int VEC, int P, int NS, bool NT, bool DPP, int K, int DLO, int DHI>
 __device__ __forceinline__ void pair_dispatch(int d, T *__restrict__ data, const StreamArgs &a, i64 i0, i64 tile,
                                               int lane, T (*head)[64 * VEC]) {
     if constexpr (DLO <= DHI) {
         if (d == DLO) {
-            pair_unit<T, WT, VEC, P, NS, NT, DPP, DLO>(data, a, i0, tile, lane, head);
+            pair_unit<T, WT, VEC, P, NS, NT, DPP, K, DLO>(data, a, i0, tile, lane, head);
             return;
         }
-        pair_dispatch<T, WT, VEC, P, NS, NT, DPP, DLO + 1, DHI>(d, data, a, i0, tile, lane, head);
+        pair_dispatch<T, WT, VEC, P, NS, NT, DPP, K, DLO + 1, DHI>(d, data, a, i0, tile, lane, head);
     }
 }
 
-// Slabs with shifts in [-SMAX, SMAX]: D in [-SMAX, SMAX-1] (a slab whose
-// only shift is SMAX runs as the pair {SMAX-1, SMAX}).
+// Slabs with shifts in [-SMAX, SMAX] (SMAX <= 2): one shift D in
+// [-SMAX, SMAX] runs the six-tap instance, two shifts {D, D+1} the seven-tap
+// instance of D in [-SMAX, SMAX-1].
 template <class T, class WT, int VEC, int SMAX, int P, int NS, int WPB, bool NT, bool DPP>
 __global__ __launch_bounds__(64 * WPB) void adv_column_pair_kernel(T *__restrict__ data, StreamArgs a) {
+    static_assert(SMAX <= 2, "six-tap instances need |D| <= 2");
     __shared__ T head_lds[WPB][NS * (SMAX + 3)][64 * VEC];
     const int lane = threadIdx.x & 63;
     const int wv = wave_uniform((int)(threadIdx.x >> 6));  // unit indices and addresses stay scalar
     for (i64 unit = (i64)blockIdx.x * WPB + wv; unit < a.total_units; unit += a.wave_stride) {
         const i64 i0 = unit / a.tiles_per_slab;
         const i64 tile = unit - i0 * a.tiles_per_slab;
-        const int d = wave_uniform(min(a.slab_dlo[i0], SMAX - 1));
-        pair_dispatch<T, WT, VEC, P, NS, NT, DPP, -SMAX, SMAX - 1>(d, data, a, i0, tile, lane, head_lds[wv]);
+        const int dlo = wave_uniform(a.slab_shift[2 * i0]);
+        const int dhi = wave_uniform(a.slab_shift[2 * i0 + 1]);
+        if (dlo == dhi)
+            pair_dispatch<T, WT, VEC, P, NS, NT, DPP, 6, -SMAX, SMAX>(dlo, data, a, i0, tile, lane, head_lds[wv]);
+        else
+            pair_dispatch<T, WT, VEC, P, NS, NT, DPP, 7, -SMAX, SMAX - 1>(dlo, data, a, i0, tile, lane,
+                                                                         head_lds[wv]);
     }
 }
 
-// Per-slab shift range with the kernels' own foot arithmetic: dlo[i0] =
-// min d, and stats = {min over slabs of min d, max over slabs of max d,
-// max over slabs of (max d - min d)} (stats preset to {INT_MAX, INT_MIN, 0}).
-__global__ void slab_shift_scan_kernel(AdvDev p, int *__restrict__ dlo, int *__restrict__ stats);
+// Per-slab shift range with the kernels' own foot arithmetic: shift[2*i0]
+// = min d, shift[2*i0+1] = max d, and stats = {min over slabs of min d, max
+// over slabs of max d, max over slabs of (max d - min d)} (stats preset to
+// {INT_MAX, INT_MIN, 0}).
+__global__ void slab_shift_scan_kernel(AdvDev p, int *__restrict__ shift, int *__restrict__ stats);
 
 
 // ---------------------------------------------------------------------------

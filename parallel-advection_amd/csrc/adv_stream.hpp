@@ -140,7 +140,7 @@ struct StreamArgs {
     i64 tiles_per_slab;   // column tiles of 64*VEC per i0 slab
     i64 total_units;      // n0 * tiles_per_slab
     i64 wave_stride;      // waves in the grid (persistent loop stride)
-    const int *slab_dlo;  // column-pair kernels: smallest shift of each i0 slab
+    const int *slab_shift;  // column-pair kernels: smallest and largest shift of each i0 slab, [n0][2]
 };
 
 // One wavefront advances unit (i0, column tile) by NS steps.
@@ -324,8 +324,11 @@ struct StreamVariant {
             &occupancy_stream<T, WT, V, S, P, W, NT, NSMAX>, 0                                                \
     }
 
-// The compiled variants of each storage type (host functions), both kernels.
+// The compiled variants of each storage type (host functions): shift-ring
+// kernels (stream_f*.hip) and column-pair kernels (pair_f*.hip).
 const StreamVariant *stream_variants_f32(int *count);
 const StreamVariant *stream_variants_f64(int *count);
+const StreamVariant *pair_variants_f32(int *count);
+const StreamVariant *pair_variants_f64(int *count);
 
 }  // namespace bkma

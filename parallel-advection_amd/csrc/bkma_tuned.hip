@@ -29,7 +29,7 @@
 
 namespace bkma {
 
-__global__ void slab_shift_scan_kernel(AdvDev p, int *__restrict__ dlo, int *__restrict__ stats) {
+__global__ void slab_shift_scan_kernel(AdvDev p, int *__restrict__ shift, int *__restrict__ stats) {
     __shared__ int smin, smax;
     const int half = (int)(p.n1 / 2);
     for (i64 i0 = blockIdx.x; i0 < p.n0; i0 += gridDim.x) {
@@ -52,7 +52,8 @@ __global__ void slab_shift_scan_kernel(AdvDev p, int *__restrict__ dlo, int *__r
         atomicMax(&smax, hi);
         __syncthreads();
         if (threadIdx.x == 0) {
-            dlo[i0] = smin;
+            shift[2 * i0] = smin;
+            shift[2 * i0 + 1] = smax;
             atomicMin(&stats[0], smin);
             atomicMax(&stats[1], smax);
             atomicMax(&stats[2], smax - smin);
@@ -66,18 +67,21 @@ __global__ void slab_shift_scan_kernel(AdvDev p, int *__restrict__ dlo, int *__r
 // ---------------------------------------------------------------------------
 namespace {
 
-// The compiled column-stream variants of a storage type.
-struct VariantSpan {
-    const StreamVariant *b;
-    int n;
-    const StreamVariant *begin() const { return b; }
-    const StreamVariant *end() const { return b + n; }
-};
-VariantSpan stream_variants(bkma_dtype dt) {
-    VariantSpan v{nullptr, 0};
-    if (dt == BKMA_F64) v.b = stream_variants_f64(&v.n);
-    if (dt == BKMA_F32) v.b = stream_variants_f32(&v.n);
-    return v;
+// The compiled column-stream and column-pair variants of a storage type.
+const std::vector<StreamVariant> &stream_variants(bkma_dtype dt) {
+    static const std::vector<StreamVariant> none;
+    auto gather = [](const StreamVariant *(*ring)(int *), const StreamVariant *(*pair)(int *)) {
+        std::vector<StreamVariant> all;
+        int n = 0;
+        const StreamVariant *t = ring(&n);
+        all.insert(all.end(), t, t + n);
+        t = pair(&n);
+        all.insert(all.end(), t, t + n);
+        return all;
+    };
+    static const std::vector<StreamVariant> f32 = gather(stream_variants_f32, pair_variants_f32);
+    static const std::vector<StreamVariant> f64 = gather(stream_variants_f64, pair_variants_f64);
+    return dt == BKMA_F32 ? f32 : dt == BKMA_F64 ? f64 : none;
 }
 
 int steps_per_pass(const bkma_tuned_config &c) { return c.steps_per_pass > 1 ? c.steps_per_pass : 1; }
@@ -108,14 +112,14 @@ struct TunedPlan {
     const StreamVariant *variant = nullptr;
     int ns = 1;                        // steps per pass
     i64 persistent_blocks[9] = {};    // by steps per pass: 0 = one wave per unit, else the grid cap
-    int *slab_dlo = nullptr;           // COLUMN_PAIR: smallest shift of each slab (owned)
+    int *slab_shift = nullptr;         // COLUMN_PAIR: smallest and largest shift of each slab, [n0][2] (owned)
     // LINE_LDS
     LdsArgs lds{};
     u64 lds_bytes = 0;
     void *halo = nullptr;  // segmented mode: halo snapshot buffer (owned)
     ~TunedPlan() {
         if (halo) (void)hipFree(halo);
-        if (slab_dlo) (void)hipFree(slab_dlo);
+        if (slab_shift) (void)hipFree(slab_shift);
     }
 };
 
@@ -149,12 +153,12 @@ bool shift_supported(const bkma_adv_params &p, std::string *why) {
 
 // Shifts of every (i0, i1) from the device's foot arithmetic: stats =
 // {smallest shift, largest shift, largest spread within one slab}; the
-// per-slab smallest shifts go to *dlo_out (device memory, owned by the
-// caller) when it is not null.
+// per-slab smallest and largest shifts ([n0][2]) go to *dlo_out (device
+// memory, owned by the caller) when it is not null.
 bkma_status scan_slab_shifts(const bkma_adv_params &p, int **dlo_out, int stats[3]) {
     int *dlo = nullptr, *dst = nullptr;
     const int init[3] = {INT_MAX, INT_MIN, 0};
-    if (hipMalloc(&dlo, std::max<u64>(p.n0, 1) * sizeof(int)) != hipSuccess) {
+    if (hipMalloc(&dlo, std::max<u64>(p.n0, 1) * 2 * sizeof(int)) != hipSuccess) {
         (void)hipGetLastError();
         return fail(BKMA_ERR_OUT_OF_MEMORY, "tuned advection: slab shift table allocation failed");
     }
@@ -311,7 +315,7 @@ bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s, int ns) 
     grid = std::min<u64>(grid, 0x7fffffffULL / (u64)(64 * v.wpb));
     if (grid == 0) return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: empty grid");
     a.wave_stride = (i64)grid * v.wpb;
-    a.slab_dlo = pl.slab_dlo;
+    a.slab_shift = pl.slab_shift;
     v.fn(data, a, (unsigned)grid, s, ns);
     return check_launch(v.pair ? "adv_column_pair_kernel" : "adv_column_stream_kernel");
 }
@@ -368,7 +372,7 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
                     int stats[3];
                     if (!shift_supported(p, &why))
                         return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
-                    BKMA_TRY(scan_slab_shifts(p, &pl->slab_dlo, stats));
+                    BKMA_TRY(scan_slab_shifts(p, &pl->slab_shift, stats));
                     if (pair_feasible(p, dt, v, ns, stats, &why) != BKMA_OK)
                         return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
                 } else if (stream_feasible(p, dt, v, ns, &why) != BKMA_OK) {

@@ -1,19 +1,16 @@
 // Compiled column-stream variants for FP32 storage (the tuner's search space).
-// Each entry instantiates steps-per-pass 1 .. NSMAX.  BKMA_STREAM_VARIANT:
-// shift-ring kernel (adv_stream.hpp); BKMA_PAIR_VARIANT: per-slab shift
-// pair kernel (adv_pair.hpp).
-#include "adv_pair.hpp"
+// Each entry instantiates steps-per-pass 1 .. NSMAX of the shift-ring kernel
+// (adv_stream.hpp); the column-pair variants are in pair_f32.hip.
 #include "adv_stream.hpp"
 
 namespace bkma {
 
-// (storage, stage-0 window, dtype, VEC, SMAX, P, waves/WG, non-temporal, [pair: DPP broadcast,] NSMAX)
+// (storage, stage-0 window, dtype, VEC, SMAX, P, waves/WG, non-temporal, NSMAX)
 // P is WS/2 or WS (WS = 2*SMAX + 6) so that one loop trip is one window long.
 const StreamVariant *stream_variants_f32(int *count) {
     static const StreamVariant table[] = {
 #ifdef BKMA_TUNED_FEW
         BKMA_STREAM_VARIANT(float, float, BKMA_F32, 4, 2, 10, 1, true, 2),
-        BKMA_PAIR_VARIANT(float, double, BKMA_F32, 4, 2, 7, 1, true, false, 3),
 #else
         BKMA_STREAM_VARIANT(float, float, BKMA_F32, 4, 2, 10, 1, true, 3),
         BKMA_STREAM_VARIANT(float, float, BKMA_F32, 4, 2, 5, 1, true, 3),
@@ -25,12 +22,6 @@ const StreamVariant *stream_variants_f32(int *count) {
         BKMA_STREAM_VARIANT(float, float, BKMA_F32, 4, 4, 7, 1, true, 2),
         BKMA_STREAM_VARIANT(float, float, BKMA_F32, 2, 4, 14, 1, true, 3),
         BKMA_STREAM_VARIANT(float, float, BKMA_F32, 1, 4, 14, 1, false, 3),
-        BKMA_PAIR_VARIANT(float, float, BKMA_F32, 4, 2, 7, 1, true, false, 4),
-        BKMA_PAIR_VARIANT(float, double, BKMA_F32, 4, 2, 7, 1, true, false, 3),
-        BKMA_PAIR_VARIANT(float, float, BKMA_F32, 2, 2, 14, 1, true, false, 5),
-        BKMA_PAIR_VARIANT(float, float, BKMA_F32, 2, 2, 14, 1, true, true, 5),
-        BKMA_PAIR_VARIANT(float, double, BKMA_F32, 2, 2, 14, 1, true, true, 5),
-        BKMA_PAIR_VARIANT(float, float, BKMA_F32, 2, 2, 7, 1, true, true, 6),
 #endif
     };
     *count = (int)(sizeof(table) / sizeof(table[0]));
@@ -38,3 +29,4 @@ const StreamVariant *stream_variants_f32(int *count) {
 }
 
 }  // namespace bkma
+

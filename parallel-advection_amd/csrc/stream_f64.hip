@@ -1,18 +1,15 @@
 // Compiled column-stream variants for FP64 storage (the tuner's search space).
-// Each entry instantiates steps-per-pass 1 .. NSMAX.  BKMA_STREAM_VARIANT:
-// shift-ring kernel (adv_stream.hpp); BKMA_PAIR_VARIANT: per-slab shift
-// pair kernel (adv_pair.hpp).
-#include "adv_pair.hpp"
+// Each entry instantiates steps-per-pass 1 .. NSMAX of the shift-ring kernel
+// (adv_stream.hpp); the column-pair variants are in pair_f64.hip.
 #include "adv_stream.hpp"
 
 namespace bkma {
 
-// (storage, stage-0 window, dtype, VEC, SMAX, P, waves/WG, non-temporal, [pair: DPP broadcast (FP32 only),] NSMAX)
+// (storage, stage-0 window, dtype, VEC, SMAX, P, waves/WG, non-temporal, NSMAX)
 const StreamVariant *stream_variants_f64(int *count) {
     static const StreamVariant table[] = {
 #ifdef BKMA_TUNED_FEW
         BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 2, 5, 1, true, 2),
-        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 2, 14, 1, true, false, 4),
 #else
         BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 2, 5, 1, true, 3),
         BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 2, 10, 1, true, 3),
@@ -20,8 +17,6 @@ const StreamVariant *stream_variants_f64(int *count) {
         BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 1, 8, 1, true, 3),
         BKMA_STREAM_VARIANT(double, double, BKMA_F64, 2, 4, 7, 1, true, 2),
         BKMA_STREAM_VARIANT(double, double, BKMA_F64, 1, 4, 14, 1, false, 3),
-        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 2, 7, 1, true, false, 6),
-        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 2, 14, 1, true, false, 5),
 #endif
     };
     *count = (int)(sizeof(table) / sizeof(table[0]));
@@ -29,3 +24,4 @@ const StreamVariant *stream_variants_f64(int *count) {
 }
 
 }  // namespace bkma
+

@@ -104,18 +104,43 @@ PAIR_VARIANTS = [(B.F32, c) for c in F32_VARIANTS if c.strategy == B.STRATEGY_CO
     [(B.F64, c) for c in F64_VARIANTS if c.strategy == B.STRATEGY_COLUMN_PAIR]
 
 
+def _slab_shift_sets(po, n0, n1):
+    """Shifts d = i1 - leftNode of every row of every slab (oracle foot)."""
+    out = []
+    for i0 in range(n0):
+        ds = set()
+        for i1 in range(n1):
+            d = i1 - O.adv_foot(po, i0, i1)[0]
+            d = d - n1 if d > n1 // 2 else d + n1 if d < -(n1 // 2) else d
+            ds.add(d)
+        out.append(tuple(sorted(ds)))
+    return out
+
+
+# (dt * n1, minRealVx, maxRealVx, n0): fractional shifts s = 2.8 v (one
+# shift per slab, d = ceil(s) in -2..2), and integer shifts s = v in -2..1.5
+# whose rows round to both sides (shift pairs {-2,-1} .. {1,2}).
+PAIR_SHIFT_CASES = [(2.8, -1.0, 0.8, 9), (1.0, -2.0, 2.0, 8)]
+
+
+@pytest.mark.parametrize("case", PAIR_SHIFT_CASES, ids=["fractional", "integer"])
 @pytest.mark.parametrize("dtype,cfg", PAIR_VARIANTS,
                          ids=[("f32-" if d == B.F32 else "f64-") + _variant_id(c) for d, c in PAIR_VARIANTS])
-def test_pair_every_shift_pair(cuda, dtype, cfg):
-    """Column-pair kernels: slabs with shifts s = v*dt/dx from -2.8 to 1.68
-    (rows shifted by d = ceil(s) in {-2, -1, 0, 1, 2}, a zero-velocity slab)
-    reach every compiled pair instance; results are bit-identical to the
-    oracle."""
-    n0, n1 = 9, 300
+def test_pair_every_shift_pair(cuda, dtype, cfg, case):
+    """Column-pair kernels: the two cases reach every compiled six-tap
+    (one shift, D in -2..2) and seven-tap (shifts {D, D+1}, D in -2..1)
+    instance; results are bit-identical to the oracle."""
+    dtn, vmin, vmax, n0 = case
+    n1 = 300
     n2 = 64 * cfg.vec
-    dt = 2.8 / n1  # dx = 1/n1, v in {-1, -0.8, ..., 0.6}
-    p = B.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=0.8)
-    po = O.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=0.8)
+    dt = dtn / n1  # dx = 1/n1
+    p = B.make_params(n0, n1, n2, dt=dt, minRealVx=vmin, maxRealVx=vmax)
+    po = O.make_params(n0, n1, n2, dt=dt, minRealVx=vmin, maxRealVx=vmax)
+    sets = _slab_shift_sets(po, n0, n1)
+    if dtn == 2.8:
+        assert {s for s in sets if len(s) == 1} == {(-2,), (-1,), (0,), (1,), (2,)}
+    else:
+        assert {s for s in sets if len(s) == 2} == {(-2, -1), (-1, 0), (0, 1), (1, 2)}
     if dtype == B.F32:
         host = O.uniform(n0 * n1 * n2, 5, np.float32).reshape(n0, n1, n2)
     else:
@@ -128,7 +153,8 @@ def test_pair_every_shift_pair(cuda, dtype, cfg):
         O.advect_f32_blocked(ref, po, nsteps, spp)
     else:
         O.advect_f64(ref, po, nsteps)
-    assert np.array_equal(got, ref), np.abs(got - ref).max()
+    bits = np.uint32 if dtype == B.F32 else np.uint64
+    assert np.array_equal(got.view(bits), ref.view(bits)), np.abs(got - ref).max()
 
 
 def test_pair_shift_beyond_range_rejected(cuda):
