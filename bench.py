@@ -30,6 +30,8 @@ N0, N1, N2 = 2048, 1024, 512
 DT = 1e-3
 SEED = 1
 HBM_PEAK_GBS = 8000.0
+FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X vector FP64
+FLOP_PER_CELL_STEP = 12  # the reference's six multiplies and six adds per cell-update
 
 
 def parse():
@@ -176,6 +178,8 @@ def main():
     bytes_alg = cells * 2 * 4  # one FP32 read and one FP32 write per cell and pass
     achieved = bytes_alg / (kern_ms32 * 1e-3) / 1e9
     per_step_gbs = gcell32 * 8.0  # bytes a one-step-per-pass kernel would move at this rate
+    # the same launch against the vector FP64 peak (the stencil sums run in FP64)
+    fp64_tflops = cells * spp * FLOP_PER_CELL_STEP / (kern_ms32 * 1e-3) / 1e12
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
 
     out = {
@@ -200,6 +204,9 @@ def main():
                      "traffic": load_traffic(args.traffic, name32) if n0 == N0 else None,
                      "kernel_avg_ms": round(kern_ms32, 5), "algorithmic_bytes_per_launch": bytes_alg,
                      "steps_per_launch": spp,
+                     "fp64_valu": {"achieved_TFLOPs": round(fp64_tflops, 2), "peak": FP64_VALU_PEAK_TFLOPS,
+                                   "frac": round(fp64_tflops / FP64_VALU_PEAK_TFLOPS, 4),
+                                   "flop_per_cell_step": FLOP_PER_CELL_STEP},
                      "per_step_equivalent": {"GBps": round(per_step_gbs, 1),
                                              "frac": round(per_step_gbs / HBM_PEAK_GBS, 4)}},
         "plain_fp64_gpu": {"value": round(gcell64, 2), "unit": "Gcell-updates/s",
