@@ -90,14 +90,12 @@ def advection_runs(torch, name, shape, steps, warmup, fp32):
         except B.BkmaError as e:
             rec["status"] = f"skipped: {e}"
             runs.append(rec)
-            _emit(rec)
             continue
         p.maxIter = steps + warmup
         rec.update(_rates(sec, cells, 8))
         rec["l1_err"] = B.validate_adv_l1(data, p)[0]
         rec["status"] = "ok"
         runs.append(rec)
-        _emit(rec)
     for dt_code, tdt, label in ((B.F64, torch.float64, "f64"), (B.F32, torch.float32, "f32")):
         if dt_code == B.F32 and not fp32:
             continue
@@ -121,19 +119,19 @@ def advection_runs(torch, name, shape, steps, warmup, fp32):
         except B.BkmaError as e:
             rec["status"] = f"skipped: {e}"
             runs.append(rec)
-            _emit(rec)
             continue
         p.maxIter = k + warmup
         rec.update(_rates(sec, cells, d.element_size()))
         rec["l1_err"] = B.validate_adv_l1(d, p)[0]
         rec["status"] = "ok"
         runs.append(rec)
-        _emit(rec)
         if d is not data:
             del d
     del data
     torch.cuda.empty_cache()
     _app_efficiency(runs)
+    for rec in runs:  # after the shape, so that each line carries its application efficiency
+        _emit(rec)
     return runs
 
 
@@ -171,13 +169,11 @@ def conv_runs(torch, steps, warmup):
         rec["result"] = B.conv_sum_normalize(data, nw)
         rec["batch_invariant"] = B.conv_validate(data, nw)
         rec["status"] = "ok"
+        rec["app_eff"] = 1.0  # one implementation per config: each is its config's best
         runs.append(rec)
         _emit(rec)
         del data
     torch.cuda.empty_cache()
-    for r in runs:  # one implementation per config: each is its config's best
-        if r.get("status") == "ok":
-            r["app_eff"] = 1.0
     return runs
 
 

@@ -45,17 +45,20 @@ constexpr int kPairTaps = 7;
 // Stage windows of K taps (K = 6: one shift D; K = 7: shifts {D, D+1}).
 // PV is the variant's prefetch depth; the six-tap instance rounds it down to
 // a multiple of six so that one loop trip stays one window long.
+// BACK may be negative (taps entirely ahead of the row, D < 4 - K): then no
+// row before row 0 is needed and R = 0.  AHEAD may be 0 (D = 3): stage t
+// then reads the row stage t-1 produced at the same position.
 template <int K, int D, int PV, int NS> struct PairShape {
     static constexpr int BACK = D + K - 4;  // rows behind the output row read by its taps
     static constexpr int AHEAD = 3 - D;     // rows in front of it (stage lag)
     static constexpr int W = K;             // rows per stage window
     static constexpr int P = K == 7 ? PV : (PV >= 12 ? 12 : 6);
     static constexpr int U = clcm(W, P);
-    static constexpr int R = NS * BACK;
+    static constexpr int R = NS * (BACK > 0 ? BACK : 0);
     static constexpr int H = NS * AHEAD;
     static constexpr int TB = NS * AHEAD;
     static_assert(K == 6 || K == 7, "six or seven taps");
-    static_assert(BACK >= 0 && AHEAD >= 1, "shift outside the supported range");
+    static_assert(AHEAD >= 0 && BACK + AHEAD == K - 1, "shift outside the supported range");
     static_assert(U + TB <= 64, "foot table must fit one wavefront");
 };
 
@@ -293,26 +296,33 @@ __device__ __forceinline__ void pair_dispatch(int d, T *__restrict__ data, const
     }
 }
 
-// Slabs with shifts in [-SMAX, SMAX] (SMAX <= 2): one shift D in
+// Slabs with shifts in [-SMAX, SMAX] (SMAX <= 3): one shift D in
 // [-SMAX, SMAX] runs the six-tap instance, two shifts {D, D+1} the seven-tap
-// instance of D in [-SMAX, SMAX-1].
+// instance of D in [-SMAX, SMAX-1].  The head rows live in dynamic LDS,
+// NS * a.pair_ahead rows per wave (pair_ahead = 3 - the smallest shift of the
+// problem, the largest stage lag of its slabs).
 template <class T, class WT, int VEC, int SMAX, int P, int NS, int WPB, bool NT, bool DPP>
 __global__ __launch_bounds__(64 * WPB) void adv_column_pair_kernel(T *__restrict__ data, StreamArgs a) {
-    static_assert(SMAX <= 2, "six-tap instances need |D| <= 2");
-    __shared__ T head_lds[WPB][NS * (SMAX + 3)][64 * VEC];
+    static_assert(SMAX <= 3, "stage lag 3 - D must stay non-negative");
+    extern __shared__ __align__(16) unsigned char pair_lds[];
     const int lane = threadIdx.x & 63;
     const int wv = wave_uniform((int)(threadIdx.x >> 6));  // unit indices and addresses stay scalar
+    T(*head)[64 * VEC] = reinterpret_cast<T(*)[64 * VEC]>(pair_lds) + (i64)wv * NS * a.pair_ahead;
     for (i64 unit = (i64)blockIdx.x * WPB + wv; unit < a.total_units; unit += a.wave_stride) {
         const i64 i0 = unit / a.tiles_per_slab;
         const i64 tile = unit - i0 * a.tiles_per_slab;
         const int dlo = wave_uniform(a.slab_shift[2 * i0]);
         const int dhi = wave_uniform(a.slab_shift[2 * i0 + 1]);
         if (dlo == dhi)
-            pair_dispatch<T, WT, VEC, P, NS, NT, DPP, 6, -SMAX, SMAX>(dlo, data, a, i0, tile, lane, head_lds[wv]);
+            pair_dispatch<T, WT, VEC, P, NS, NT, DPP, 6, -SMAX, SMAX>(dlo, data, a, i0, tile, lane, head);
         else
-            pair_dispatch<T, WT, VEC, P, NS, NT, DPP, 7, -SMAX, SMAX - 1>(dlo, data, a, i0, tile, lane,
-                                                                         head_lds[wv]);
+            pair_dispatch<T, WT, VEC, P, NS, NT, DPP, 7, -SMAX, SMAX - 1>(dlo, data, a, i0, tile, lane, head);
     }
+}
+
+// Dynamic LDS of one work-group of a pair variant: the head rows of its waves.
+template <class T, int VEC, int WPB> inline size_t pair_lds_bytes(int ns, int ahead) {
+    return (size_t)WPB * ns * (ahead > 0 ? ahead : 0) * 64 * VEC * sizeof(T) + 16;
 }
 
 // Per-slab shift range with the kernels' own foot arithmetic: shift[2*i0]
@@ -329,25 +339,27 @@ template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, bool DP
 void launch_pair(void *data, const StreamArgs &a, unsigned grid, hipStream_t s, int ns) {
     if constexpr (NS >= 1) {
         if (ns == NS) {
+            const size_t lds = pair_lds_bytes<T, VEC, WPB>(NS, a.pair_ahead);
             hipLaunchKernelGGL((adv_column_pair_kernel<T, WT, VEC, SMAX, P, NS, WPB, NT, DPP>), dim3(grid),
-                               dim3(64 * WPB), 0, s, reinterpret_cast<T *>(data), a);
+                               dim3(64 * WPB), lds, s, reinterpret_cast<T *>(data), a);
             return;
         }
         launch_pair<T, WT, VEC, SMAX, P, WPB, NT, DPP, NS - 1>(data, a, grid, s, ns);
     }
 }
 
-template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, bool DPP, int NS> int occupancy_pair(int ns) {
+template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, bool DPP, int NS>
+int occupancy_pair(int ns, int ahead) {
     if constexpr (NS >= 1) {
         if (ns == NS) {
             int blocks = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &blocks, adv_column_pair_kernel<T, WT, VEC, SMAX, P, NS, WPB, NT, DPP>, 64 * WPB, 0) !=
-                hipSuccess)
+                    &blocks, adv_column_pair_kernel<T, WT, VEC, SMAX, P, NS, WPB, NT, DPP>, 64 * WPB,
+                    pair_lds_bytes<T, VEC, WPB>(NS, ahead)) != hipSuccess)
                 return 0;
             return blocks;
         }
-        return occupancy_pair<T, WT, VEC, SMAX, P, WPB, NT, DPP, NS - 1>(ns);
+        return occupancy_pair<T, WT, VEC, SMAX, P, WPB, NT, DPP, NS - 1>(ns, ahead);
     }
     return 0;
 }

@@ -141,6 +141,7 @@ struct StreamArgs {
     i64 total_units;      // n0 * tiles_per_slab
     i64 wave_stride;      // waves in the grid (persistent loop stride)
     const int *slab_shift;  // column-pair kernels: smallest and largest shift of each i0 slab, [n0][2]
+    int pair_ahead;         // column-pair kernels: head rows per fused step (3 - smallest shift)
 };
 
 // One wavefront advances unit (i0, column tile) by NS steps.
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(64 * WPB) void adv_column_stream_kernel(T *__restri
 // Host side: launchers for steps-per-pass 1 .. NSMAX of one variant
 // ---------------------------------------------------------------------------
 using StreamLaunchFn = void (*)(void *data, const StreamArgs &a, unsigned grid, hipStream_t s, int ns);
-using StreamOccFn = int (*)(int ns);
+using StreamOccFn = int (*)(int ns, int pair_ahead);
 
 template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, int NS>
 void launch_stream(void *data, const StreamArgs &a, unsigned grid, hipStream_t s, int ns) {
@@ -294,7 +295,8 @@ void launch_stream(void *data, const StreamArgs &a, unsigned grid, hipStream_t s
 }
 
 // Resident work-groups per CU (0 if the query fails or ns is not compiled).
-template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, int NS> int occupancy_stream(int ns) {
+template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, int NS>
+int occupancy_stream(int ns, int pair_ahead = 0) {
     if constexpr (NS >= 1) {
         if (ns == NS) {
             int blocks = 0;
@@ -303,7 +305,7 @@ template <class T, class WT, int VEC, int SMAX, int P, int WPB, bool NT, int NS>
                 return 0;
             return blocks;
         }
-        return occupancy_stream<T, WT, VEC, SMAX, P, WPB, NT, NS - 1>(ns);
+        return occupancy_stream<T, WT, VEC, SMAX, P, WPB, NT, NS - 1>(ns, pair_ahead);
     }
     return 0;
 }

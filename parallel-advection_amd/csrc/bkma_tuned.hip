@@ -113,6 +113,7 @@ struct TunedPlan {
     int ns = 1;                        // steps per pass
     i64 persistent_blocks[9] = {};    // by steps per pass: 0 = one wave per unit, else the grid cap
     int *slab_shift = nullptr;         // COLUMN_PAIR: smallest and largest shift of each slab, [n0][2] (owned)
+    int pair_ahead = 0;                // COLUMN_PAIR: largest stage lag, 3 - the smallest shift
     // LINE_LDS
     LdsArgs lds{};
     u64 lds_bytes = 0;
@@ -316,6 +317,7 @@ bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s, int ns) 
     if (grid == 0) return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: empty grid");
     a.wave_stride = (i64)grid * v.wpb;
     a.slab_shift = pl.slab_shift;
+    a.pair_ahead = pl.pair_ahead;
     v.fn(data, a, (unsigned)grid, s, ns);
     return check_launch(v.pair ? "adv_column_pair_kernel" : "adv_column_stream_kernel");
 }
@@ -344,11 +346,11 @@ std::string lds_name(bkma_dtype dt, const LdsArgs &a) {
 }
 
 // Resident work-groups on the whole device for a variant (persistent grid).
-i64 resident_blocks(const StreamVariant &v, int ns) {
+i64 resident_blocks(const StreamVariant &v, int ns, int pair_ahead) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    return (i64)v.occ(ns) * cus;
+    return (i64)v.occ(ns, pair_ahead) * cus;
 }
 
 // A candidate plan (not yet owning device memory).
@@ -375,13 +377,15 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
                     BKMA_TRY(scan_slab_shifts(p, &pl->slab_shift, stats));
                     if (pair_feasible(p, dt, v, ns, stats, &why) != BKMA_OK)
                         return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
+                    pl->pair_ahead = std::max(0, 3 - stats[0]);
                 } else if (stream_feasible(p, dt, v, ns, &why) != BKMA_OK) {
                     return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
                 }
                 pl->variant = &v;
                 pl->ns = ns;
                 pl->cfg.steps_per_pass = ns;
-                for (int k = 1; k <= ns; ++k) pl->persistent_blocks[k] = cfg.persistent ? resident_blocks(v, k) : 0;
+                for (int k = 1; k <= ns; ++k)
+                    pl->persistent_blocks[k] = cfg.persistent ? resident_blocks(v, k, pl->pair_ahead) : 0;
                 pl->cfg.persistent = pl->persistent_blocks[ns] > 0;
                 pl->name = stream_name(v, ns, pl->persistent_blocks[ns] > 0);
                 return BKMA_OK;

@@ -9,12 +9,10 @@ namespace bkma {
 const StreamVariant *pair_variants_f32(int *count) {
     static const StreamVariant table[] = {
 #ifdef BKMA_TUNED_FEW
-        BKMA_PAIR_VARIANT(float, double, BKMA_F32, 2, 2, 14, 1, true, true, 5),
+        BKMA_PAIR_VARIANT(float, double, BKMA_F32, 2, 3, 14, 1, true, true, 5),
 #else
-        BKMA_PAIR_VARIANT(float, double, BKMA_F32, 4, 2, 7, 1, true, false, 3),
-        BKMA_PAIR_VARIANT(float, double, BKMA_F32, 2, 2, 14, 1, true, true, 6),
-        BKMA_PAIR_VARIANT(float, float, BKMA_F32, 2, 2, 14, 1, true, true, 6),
-        BKMA_PAIR_VARIANT(float, double, BKMA_F32, 2, 2, 7, 1, true, true, 6),
+        BKMA_PAIR_VARIANT(float, double, BKMA_F32, 4, 3, 7, 1, true, false, 3),
+        BKMA_PAIR_VARIANT(float, double, BKMA_F32, 2, 3, 14, 1, true, true, 5),
 #endif
     };
     *count = (int)(sizeof(table) / sizeof(table[0]));

@@ -9,10 +9,9 @@ namespace bkma {
 const StreamVariant *pair_variants_f64(int *count) {
     static const StreamVariant table[] = {
 #ifdef BKMA_TUNED_FEW
-        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 2, 14, 1, true, false, 4),
+        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 3, 14, 1, true, false, 4),
 #else
-        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 2, 7, 1, true, false, 6),
-        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 2, 14, 1, true, false, 5),
+        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 3, 14, 1, true, false, 5),
 #endif
     };
     *count = (int)(sizeof(table) / sizeof(table[0]));

@@ -118,18 +118,19 @@ def _slab_shift_sets(po, n0, n1):
 
 
 # (dt * n1, minRealVx, maxRealVx, n0): fractional shifts s = 2.8 v (one
-# shift per slab, d = ceil(s) in -2..2), and integer shifts s = v in -2..1.5
-# whose rows round to both sides (shift pairs {-2,-1} .. {1,2}).
-PAIR_SHIFT_CASES = [(2.8, -1.0, 0.8, 9), (1.0, -2.0, 2.0, 8)]
+# shift per slab, d = ceil(s) in -2..2), and shifts s = v in -3.5..2.5 by
+# halves: the half-integers give one shift per slab (-3..3), the integers
+# rows that round to both sides (shift pairs {-3,-2} .. {2,3}).
+PAIR_SHIFT_CASES = [(2.8, -1.0, 0.8, 9), (1.0, -3.5, 3.0, 13)]
 
 
 @pytest.mark.parametrize("case", PAIR_SHIFT_CASES, ids=["fractional", "integer"])
 @pytest.mark.parametrize("dtype,cfg", PAIR_VARIANTS,
                          ids=[("f32-" if d == B.F32 else "f64-") + _variant_id(c) for d, c in PAIR_VARIANTS])
 def test_pair_every_shift_pair(cuda, dtype, cfg, case):
-    """Column-pair kernels: the two cases reach every compiled six-tap
-    (one shift, D in -2..2) and seven-tap (shifts {D, D+1}, D in -2..1)
-    instance; results are bit-identical to the oracle."""
+    """Column-pair kernels: the cases reach every compiled six-tap (one
+    shift, D in -3..3) and seven-tap (shifts {D, D+1}, D in -3..2) instance;
+    results are bit-identical to the oracle."""
     dtn, vmin, vmax, n0 = case
     n1 = 300
     n2 = 64 * cfg.vec
@@ -140,7 +141,8 @@ def test_pair_every_shift_pair(cuda, dtype, cfg, case):
     if dtn == 2.8:
         assert {s for s in sets if len(s) == 1} == {(-2,), (-1,), (0,), (1,), (2,)}
     else:
-        assert {s for s in sets if len(s) == 2} == {(-2, -1), (-1, 0), (0, 1), (1, 2)}
+        assert {s for s in sets if len(s) == 1} == {(d,) for d in range(-3, 4)}
+        assert {s for s in sets if len(s) == 2} == {(d, d + 1) for d in range(-3, 3)}
     if dtype == B.F32:
         host = O.uniform(n0 * n1 * n2, 5, np.float32).reshape(n0, n1, n2)
     else:
@@ -160,7 +162,7 @@ def test_pair_every_shift_pair(cuda, dtype, cfg, case):
 def test_pair_shift_beyond_range_rejected(cuda):
     cfg = next(c for c in F32_VARIANTS if c.strategy == B.STRATEGY_COLUMN_PAIR)
     n1 = 300
-    p = B.make_params(2, n1, 256, dt=2.5 / n1, minRealVx=0, maxRealVx=2)  # s = 2.5: d = 3
+    p = B.make_params(2, n1, 256, dt=3.5 / n1, minRealVx=0, maxRealVx=2)  # s = 3.5: d = 4
     with pytest.raises(B.BkmaUnsupported):
         B.AdvPlan(p, B.F32, config=cfg)
 
