@@ -184,8 +184,10 @@ bkma_status bkma_conv_run(bkma_impl impl, bkma_dtype dtype, void *data, uint64_t
 typedef enum bkma_strategy {
     BKMA_STRATEGY_COLUMN_STREAM = 1, /* lines strided by n2 (n2 >= 64), small shifts: register ring per wavefront */
     BKMA_STRATEGY_LINE_LDS = 2,      /* any n2 and shift: slab segments + halos staged in LDS */
-    BKMA_STRATEGY_COLUMN_PAIR = 3    /* as COLUMN_STREAM, for slabs whose rows take at most two adjacent shifts
+    BKMA_STRATEGY_COLUMN_PAIR = 3,   /* as COLUMN_STREAM, for slabs whose rows take at most two adjacent shifts
                                         (checked per slab at plan creation): seven-row windows, no shift branches */
+    BKMA_STRATEGY_PLAIN = 4          /* the reference's AdaptiveWg executor and create_optim_params dispatch
+                                        (F64 storage, line in LDS; seg_rows = pref_wg_size), one step per pass */
 } bkma_strategy;
 
 typedef struct bkma_tuned_config {
@@ -196,7 +198,7 @@ typedef struct bkma_tuned_config {
     int32_t waves_per_wg; /* wavefronts per work-group */
     int32_t nontemporal;  /* 1: non-temporal global loads/stores */
     int32_t persistent;   /* 1: grid capped at the resident work-groups, waves loop over units */
-    int32_t seg_rows;     /* LINE_LDS: rows per segment (0 = whole line in LDS) */
+    int32_t seg_rows;     /* LINE_LDS: rows per segment (0 = whole line in LDS); PLAIN: pref_wg_size */
     int32_t wide_window;  /* COLUMN_STREAM, F32 storage: 1 = FP64 register window (rows widened once) */
     int32_t steps_per_pass; /* COLUMN_STREAM: time steps fused into one pass over HBM (0 or 1: one) */
     int32_t chunk_segs;   /* LINE_LDS segmented: consecutive segments one work-group walks in order (0 or 1: one) */

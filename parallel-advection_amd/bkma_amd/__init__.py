@@ -106,6 +106,7 @@ class TunedConfig(ctypes.Structure):
 STRATEGY_COLUMN_STREAM = 1
 STRATEGY_LINE_LDS = 2
 STRATEGY_COLUMN_PAIR = 3
+STRATEGY_PLAIN = 4
 
 
 class AdvProblem(ctypes.Structure):

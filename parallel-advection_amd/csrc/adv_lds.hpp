@@ -41,9 +41,9 @@ __device__ __forceinline__ void slab_halo(const AdvDev &p, i64 i0, i64 hb_max, i
 // halo[(i0 * nseg + seg) * (hb_max + ha_max) * n2 ...]: first the hb rows
 // before the segment, then the ha rows after it (periodic in i1).
 template <class T>
-__global__ void halo_snapshot_kernel(const T *__restrict__ data, T *__restrict__ halo, LdsArgs a) {
+__device__ __forceinline__ void halo_snapshot_unit(const T *__restrict__ data, T *__restrict__ halo,
+                                                   const LdsArgs &a, i64 unit) {
     const AdvDev &p = a.p;
-    const i64 unit = blockIdx.x;
     const i64 i0 = unit / a.nseg, seg = unit - i0 * a.nseg;
     const i64 n1 = p.n1, n2 = p.n2;
     const i64 r0 = seg * a.seg_rows;
@@ -60,6 +60,14 @@ __global__ void halo_snapshot_kernel(const T *__restrict__ data, T *__restrict__
         if (row < 0) row += n1;
         dst[e] = slab[row * n2 + c];
     }
+}
+
+// Grid-stride over the (slab, segment) units: a grid is capped at
+// kLdsMaxGrid work-groups (the launch limit is 2^32 work-items).
+constexpr i64 kLdsMaxGrid = (i64)1 << 22;
+template <class T>
+__global__ void halo_snapshot_kernel(const T *__restrict__ data, T *__restrict__ halo, LdsArgs a) {
+    for (i64 unit = blockIdx.x; unit < a.p.n0 * a.nseg; unit += gridDim.x) halo_snapshot_unit(data, halo, a, unit);
 }
 
 // Outputs of rows [r0, r0+nr) from the staged rows in LDS.  Thread (row
@@ -185,12 +193,15 @@ template <class T, bool SEG>
 __global__ __launch_bounds__(256) void adv_line_lds_kernel(T *__restrict__ data, const T *__restrict__ halo,
                                                            LdsArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    const i64 unit = blockIdx.x;
-    const i64 i0 = unit / a.nseg, seg = unit - i0 * a.nseg;
-    const i64 r0 = seg * a.seg_rows;
-    const i64 r1 = min(a.p.n1, r0 + a.seg_rows);
-    line_lds_unit<T, SEG>(data + i0 * a.p.n1 * a.p.n2, SEG ? halo + unit * (a.hb_max + a.ha_max) * a.p.n2 : nullptr,
-                          a.p, i0, r0, r1, a.hb_max, a.ha_max, reinterpret_cast<T *>(lds_raw));
+    for (i64 unit = blockIdx.x; unit < a.p.n0 * a.nseg; unit += gridDim.x) {
+        const i64 i0 = unit / a.nseg, seg = unit - i0 * a.nseg;
+        const i64 r0 = seg * a.seg_rows;
+        const i64 r1 = min(a.p.n1, r0 + a.seg_rows);
+        line_lds_unit<T, SEG>(data + i0 * a.p.n1 * a.p.n2,
+                              SEG ? halo + unit * (a.hb_max + a.ha_max) * a.p.n2 : nullptr, a.p, i0, r0, r1,
+                              a.hb_max, a.ha_max, reinterpret_cast<T *>(lds_raw));
+        __syncthreads();  // the next unit restages the LDS
+    }
 }
 
 // Rows [c0, c1) of slab i0 ("chunk"), walked by one work-group in segments
@@ -252,13 +263,15 @@ template <class T>
 __global__ __launch_bounds__(256) void adv_line_chunk_kernel(T *__restrict__ data, const T *__restrict__ halo,
                                                              LdsArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    const i64 unit = blockIdx.x;
-    const i64 i0 = unit / a.nchunk, ch = unit - i0 * a.nchunk;
-    const i64 chunk_rows = a.seg_rows * a.chunk_segs;
-    const i64 c0 = ch * chunk_rows;
-    const i64 c1 = min(a.p.n1, c0 + chunk_rows);
-    line_chunk_unit<T>(data + i0 * a.p.n1 * a.p.n2, halo + unit * (a.hb_max + a.ha_max) * a.p.n2, a.p, i0, c0, c1,
-                       a.seg_rows, a.hb_max, a.ha_max, reinterpret_cast<T *>(lds_raw));
+    for (i64 unit = blockIdx.x; unit < a.p.n0 * a.nchunk; unit += gridDim.x) {
+        const i64 i0 = unit / a.nchunk, ch = unit - i0 * a.nchunk;
+        const i64 chunk_rows = a.seg_rows * a.chunk_segs;
+        const i64 c0 = ch * chunk_rows;
+        const i64 c1 = min(a.p.n1, c0 + chunk_rows);
+        line_chunk_unit<T>(data + i0 * a.p.n1 * a.p.n2, halo + unit * (a.hb_max + a.ha_max) * a.p.n2, a.p, i0, c0,
+                           c1, a.seg_rows, a.hb_max, a.ha_max, reinterpret_cast<T *>(lds_raw));
+        __syncthreads();  // the next unit restages the LDS
+    }
 }
 
 }  // namespace bkma

@@ -113,6 +113,7 @@ struct TunedPlan {
     int ns = 1;                        // steps per pass
     i64 persistent_blocks[9] = {};    // by steps per pass: 0 = one wave per unit, else the grid cap
     int *slab_shift = nullptr;         // COLUMN_PAIR: smallest and largest shift of each slab, [n0][2] (owned)
+    bkma_optim_params optim{};         // PLAIN: the reference's dispatch for its AdaptiveWg executor
     int pair_ahead = 0;                // COLUMN_PAIR: largest stage lag, 3 - the smallest shift
     // LINE_LDS
     LdsArgs lds{};
@@ -131,8 +132,8 @@ bool params_finite(const bkma_adv_params &p) {
            std::isfinite(p.minRealX) && p.realWidthX > 0 && std::isfinite(p.realWidthX);
 }
 
-// Every tuned strategy assumes |v*dt/dx| well below n1/2 (the reference's
-// unsigned stencil index is then the plain modulus).
+// Every tuned strategy assumes |v*dt/dx| + 4 below n1/2: a stencil offset
+// from its row, normalised into (-n1/2, n1/2], is then the true one.
 bool shift_supported(const bkma_adv_params &p, std::string *why) {
     if (!params_finite(p)) {
         if (why) *why = "non-finite parameters";
@@ -144,7 +145,7 @@ bool shift_supported(const bkma_adv_params &p, std::string *why) {
     }
     int dlo, dhi;
     const double smax = std::fabs(p.dt * p.inv_dx) * std::max(std::fabs(p.minRealVx), std::fabs(p.maxRealVx));
-    if (!(smax < (double)p.n1 / 2 - 8)) {
+    if (!(smax + 4 < (double)p.n1 / 2)) {
         if (why) *why = "shift of half a line or more";
         return false;
     }
@@ -281,13 +282,13 @@ bkma_status lds_shape(const bkma_adv_params &p, bkma_dtype dt, i64 seg_rows, i64
 template <class T>
 bkma_status launch_lds(const TunedPlan &pl, void *data, hipStream_t s) {
     const LdsArgs &a = pl.lds;
-    const unsigned units = (unsigned)((i64)pl.params.n0 * a.nseg);
+    const unsigned units = (unsigned)std::min<i64>((i64)pl.params.n0 * a.nseg, kLdsMaxGrid);
     if (a.nseg > 1) {
         // halo snapshot per chunk: the chunk's rows seen as one long segment
         LdsArgs sa = a;
         sa.seg_rows = a.seg_rows * a.chunk_segs;
         sa.nseg = a.nchunk;
-        const unsigned cunits = (unsigned)((i64)pl.params.n0 * a.nchunk);
+        const unsigned cunits = (unsigned)std::min<i64>((i64)pl.params.n0 * a.nchunk, kLdsMaxGrid);
         hipLaunchKernelGGL(halo_snapshot_kernel<T>, dim3(cunits), dim3(256), 0, s, (const T *)data, (T *)pl.halo, sa);
         BKMA_TRY(check_launch("halo_snapshot_kernel"));
         auto k = adv_line_chunk_kernel<T>;
@@ -303,6 +304,10 @@ bkma_status launch_lds(const TunedPlan &pl, void *data, hipStream_t s) {
 
 // One pass of ns steps (ns <= the plan's steps per pass).
 bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s, int ns) {
+    if (pl.cfg.strategy == BKMA_STRATEGY_PLAIN) {
+        const bkma_span3d none{nullptr, 0, 0, 0};
+        return bkma_adv_run(BKMA_IMPL_ADAPTIVE_WG, pl.dtype, data, &pl.params, &pl.optim, none, s);
+    }
     if (pl.cfg.strategy == BKMA_STRATEGY_LINE_LDS)
         return pl.dtype == BKMA_F64 ? launch_lds<double>(pl, data, s) : launch_lds<float>(pl, data, s);
     if (ns < 1 || ns > pl.variant->nsmax) return fail(BKMA_ERR_RUNTIME, "tuned advection: steps per pass out of range");
@@ -393,6 +398,18 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
         }
         return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: no compiled variant matches the configuration");
     }
+    if (cfg.strategy == BKMA_STRATEGY_PLAIN) {
+        // FP64 only: the plain executor's FP32 storage sums with separately
+        // rounded multiply and add, the tuned FP32 plans with fused ones
+        if (dt != BKMA_F64) return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: PLAIN runs FP64 storage only");
+        if (steps_per_pass(cfg) != 1) return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: PLAIN runs one step per pass");
+        pl->cfg.steps_per_pass = 1;
+        BKMA_TRY(bkma_create_optim_params(p.n0, p.n1, p.n2, (u64)cfg.seg_rows, 1, 1, 8, 0, &pl->optim));
+        if (pl->optim.mem_space != BKMA_MEM_LOCAL)
+            return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: PLAIN needs the line in LDS");
+        pl->name = "plain_adaptivewg<pref" + std::to_string(cfg.seg_rows) + ">";
+        return BKMA_OK;
+    }
     if (cfg.strategy == BKMA_STRATEGY_LINE_LDS) {
         if (steps_per_pass(cfg) != 1)
             return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: LINE_LDS runs one step per pass");
@@ -481,6 +498,22 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
             if (!dup) out.push_back(c);
         }
     }
+    // the reference's AdaptiveWg executor and dispatch policy (FP64 storage,
+    // line in LDS): plans never time slower than it
+    if (dt == BKMA_F64) {
+        for (int pref : {128, 256, 1024}) {
+            bkma_optim_params o;
+            if (bkma_create_optim_params(p.n0, p.n1, p.n2, (u64)pref, 1, 1, 8, 0, &o) != BKMA_OK ||
+                o.mem_space != BKMA_MEM_LOCAL)
+                continue;
+            bkma_tuned_config c;
+            std::memset(&c, 0, sizeof(c));
+            c.strategy = BKMA_STRATEGY_PLAIN;
+            c.steps_per_pass = 1;
+            c.seg_rows = pref;
+            out.push_back(c);
+        }
+    }
     return out;
 }
 
@@ -490,7 +523,9 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
 // 3 steps per pass; more steps push the register window past the occupancy
 // that hides HBM latency), widest vector, deepest prefetch; else LINE_LDS
 // whole line, else the longest segment.
-int heuristic_score(const bkma_tuned_config &c, bkma_dtype dt) {
+int heuristic_score(const bkma_tuned_config &c, bkma_dtype dt, const bkma_adv_params &p) {
+    if (c.strategy == BKMA_STRATEGY_PLAIN)  // short lines: one work-group per slab would idle
+        return (p.n1 < 64 ? 900000000 : 10000000) - std::abs(c.seg_rows - 256);
     if (c.strategy == BKMA_STRATEGY_COLUMN_PAIR) {
         // FP32: DPP broadcast, FP64 stage-0 window, VEC 2, P 14, 5 steps;
         // FP64: VEC 2, P 14, 4 steps
@@ -550,7 +585,7 @@ extern "C" bkma_status bkma_adv_plan_create(const bkma_adv_params *params, bkma_
     if (cands.empty())
         return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: no strategy covers this shape (" + why + ")");
     bkma_tuned_config best = *std::max_element(cands.begin(), cands.end(), [&](const auto &a, const auto &b) {
-        return heuristic_score(a, dtype) < heuristic_score(b, dtype);
+        return heuristic_score(a, dtype, *params) < heuristic_score(b, dtype, *params);
     });
 
     if (autotune && cands.size() > 1) {

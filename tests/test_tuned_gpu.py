@@ -167,6 +167,53 @@ def test_pair_shift_beyond_range_rejected(cuda):
         B.AdvPlan(p, B.F32, config=cfg)
 
 
+def _plain_config(pref):
+    c = B.TunedConfig()
+    c.strategy = B.STRATEGY_PLAIN
+    c.seg_rows = pref
+    c.steps_per_pass = 1
+    return c
+
+
+@pytest.mark.parametrize("pref", [128, 1024])
+def test_plain_strategy_bit_exact(cuda, pref):
+    """PLAIN runs the reference's AdaptiveWg executor inside a plan."""
+    n0, n1, n2 = 5, 300, 24
+    p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    po = O.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    host = O.uniform(n0 * n1 * n2, 8).reshape(n0, n1, n2)
+    got, spp = _run(host, p, B.F64, 4, config=_plain_config(pref))
+    assert spp == 1
+    ref = host.copy()
+    O.advect_f64(ref, po, 4)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+def test_plain_strategy_is_fp64_only(cuda):
+    p = B.make_params(5, 300, 24, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    with pytest.raises(B.BkmaUnsupported):
+        B.AdvPlan(p, B.F32, config=_plain_config(256))
+
+
+@pytest.mark.parametrize("dtype", [B.F64, B.F32])
+def test_short_lines(cuda, dtype):
+    """Lines of 16 rows (the reference benchmark's n1 = 16 shapes): the
+    autotuned plan is bit-identical to the oracle."""
+    n0, n1, n2 = 64, 16, 8
+    p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    po = O.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    np_dt = np.float64 if dtype == B.F64 else np.float32
+    host = O.uniform(n0 * n1 * n2, 9, np_dt).reshape(n0, n1, n2)
+    got, spp = _run(host, p, dtype, 6, autotune=True)
+    ref = host.copy()
+    if dtype == B.F64:
+        O.advect_f64(ref, po, 6)
+    else:
+        O.advect_f32_blocked(ref, po, 6, spp)
+    bits = np.uint64 if dtype == B.F64 else np.uint32
+    assert np.array_equal(got.view(bits), ref.view(bits))
+
+
 def test_integer_shift_known_answer(cuda):
     # v*dt/dx integer (dt = 1/512, n1 = 1024, v = -1 + i0/2): circular shift
     n0, n1, n2 = 4, 1024, 128
