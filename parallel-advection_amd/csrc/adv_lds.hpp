@@ -5,7 +5,10 @@
 // staged in LDS, then every output of the segment is computed from LDS and
 // written once to HBM.
 //  - Whole-line mode (one segment per slab): the halo is the line's own wrap
-//    around, read with modular indexing; nothing else is needed.
+//    around, read with modular indexing; nothing else is needed.  This mode
+//    can fuse ns steps per pass (line_steps_unit): the staged line is
+//    advanced ns times inside LDS, the intermediate steps in FP64 work
+//    buffers, and only the last step is written to HBM.
 //  - Segmented mode (lines longer than the LDS budget): other work-groups
 //    overwrite the rows around a segment concurrently, so the halo rows are
 //    first copied to a scratch buffer by halo_snapshot_kernel; the halo of
@@ -26,6 +29,7 @@ struct LdsArgs {
     i64 ha_max;      // halo rows after a segment (allocation bound)
     i64 chunk_segs;  // segmented mode: consecutive segments one work-group walks in order
     i64 nchunk;      // chunks per slab
+    i64 ns;          // whole-line mode: steps per pass (1: line_lds_unit)
 };
 
 // Halo extents of slab i0: rows [r0 - hb, r0) and [r1, r1 + ha).
@@ -189,6 +193,86 @@ __device__ __forceinline__ void line_lds_unit(T *__restrict__ slab, const T *__r
         lds_compute<T, SEG, 1>(slab, p, i0, r0, nr, hb, lds_rows, lds);
 }
 
+// One step of a whole staged line: dst = step(src), both holding all n1 rows
+// of n2 columns (src in LDS, dst in LDS or the slab in HBM).  Arithmetic of
+// the storage type T: FP64 storage sums with separately rounded multiply and
+// add, FP32 storage with fused ones, in FP64 either way.  Thread (row offset,
+// group of VC columns); the foot is computed once per (row, group).
+template <class T, int VC, class S, class D>
+__device__ __forceinline__ void line_step(const S *__restrict__ src, D *__restrict__ dst, const AdvDev &p, i64 i0) {
+    const i64 n1 = p.n1, n2 = p.n2;
+    const int nthr = blockDim.x;
+    const i64 nch = n2 / VC;
+    const int cw = (int)min<i64>(nch, nthr);
+    const int rows_per_pass = nthr / cw;
+    const int t_row = threadIdx.x / cw, t_col = threadIdx.x - t_row * cw;
+    if (t_row >= rows_per_pass) return;
+    double coef[6];
+    unsigned long long cached = ~0ull;
+    for (i64 i1 = t_row; i1 < n1; i1 += rows_per_pass) {
+        double d_prev1;
+        const int leftNode = adv_foot(p, (int)i0, (int)i1, d_prev1);
+        const unsigned long long bits = __builtin_bit_cast(unsigned long long, d_prev1);
+        if (bits != cached) {
+            adv_lag_basis(d_prev1, coef);
+            cached = bits;
+        }
+        const int ipos1 = leftNode - 2;
+        const i64 base = stencil_fast_ok(n1, ipos1) ? stencil_base(n1, ipos1) : stencil_index_slow(n1, ipos1, 0);
+        for (i64 ch = t_col; ch < nch; ch += cw) {
+            double acc[VC];
+#pragma unroll
+            for (int v = 0; v < VC; ++v) acc[v] = 0.;
+            i64 r = base;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+#pragma unroll
+                for (int v = 0; v < VC; ++v) {
+                    const double f = (double)src[r * n2 + ch * VC + v];
+                    if constexpr (sizeof(T) == 8) acc[v] = acc[v] + coef[k] * f;
+                    else acc[v] = fma(coef[k], f, acc[v]);
+                }
+                r += 1;
+                if (r >= n1) r -= n1;
+            }
+#pragma unroll
+            for (int v = 0; v < VC; ++v) dst[i1 * n2 + ch * VC + v] = (D)acc[v];
+        }
+    }
+}
+
+// Whole line of slab i0 advanced by ns steps in one pass: staged once into
+// `lds` (storage type), steps 1 .. ns-1 into the FP64 buffers w0, w1 in
+// turn, step ns from LDS to the slab in HBM.  Every thread calls it; it has
+// no early return.
+template <class T>
+__device__ __forceinline__ void line_steps_unit(T *__restrict__ slab, const AdvDev &p, i64 i0, int ns,
+                                                T *__restrict__ lds, double *__restrict__ w0,
+                                                double *__restrict__ w1) {
+    const i64 cnt = p.n1 * p.n2;
+    for (i64 e = threadIdx.x; e < cnt; e += blockDim.x) lds[e] = slab[e];
+    __syncthreads();
+    auto step = [&](const auto *src, auto *dst) __attribute__((always_inline)) {
+        if (p.n2 % 2 == 0)
+            line_step<T, 2>(src, dst, p, i0);
+        else
+            line_step<T, 1>(src, dst, p, i0);
+        __syncthreads();
+    };
+    if (ns == 1) {
+        step(lds, slab);
+        return;
+    }
+    step(lds, w0);
+    for (int t = 2; t < ns; ++t) {
+        if (t % 2 == 0)
+            step((const double *)w0, w1);
+        else
+            step((const double *)w1, w0);
+    }
+    step((const double *)(ns % 2 == 0 ? w0 : w1), slab);
+}
+
 template <class T, bool SEG>
 __global__ __launch_bounds__(256) void adv_line_lds_kernel(T *__restrict__ data, const T *__restrict__ halo,
                                                            LdsArgs a) {
@@ -197,9 +281,17 @@ __global__ __launch_bounds__(256) void adv_line_lds_kernel(T *__restrict__ data,
         const i64 i0 = unit / a.nseg, seg = unit - i0 * a.nseg;
         const i64 r0 = seg * a.seg_rows;
         const i64 r1 = min(a.p.n1, r0 + a.seg_rows);
-        line_lds_unit<T, SEG>(data + i0 * a.p.n1 * a.p.n2,
-                              SEG ? halo + unit * (a.hb_max + a.ha_max) * a.p.n2 : nullptr, a.p, i0, r0, r1,
-                              a.hb_max, a.ha_max, reinterpret_cast<T *>(lds_raw));
+        if (!SEG && a.ns > 1) {
+            // whole line, ns steps: [n1*n2] T, then two [n1*n2] FP64 buffers
+            const i64 cnt = a.p.n1 * a.p.n2;
+            T *st = reinterpret_cast<T *>(lds_raw);
+            double *w0 = reinterpret_cast<double *>(lds_raw + ((cnt * (i64)sizeof(T) + 15) & ~(i64)15));
+            line_steps_unit<T>(data + i0 * cnt, a.p, i0, (int)a.ns, st, w0, w0 + cnt);
+        } else {
+            line_lds_unit<T, SEG>(data + i0 * a.p.n1 * a.p.n2,
+                                  SEG ? halo + unit * (a.hb_max + a.ha_max) * a.p.n2 : nullptr, a.p, i0, r0, r1,
+                                  a.hb_max, a.ha_max, reinterpret_cast<T *>(lds_raw));
+        }
         __syncthreads();  // the next unit restages the LDS
     }
 }

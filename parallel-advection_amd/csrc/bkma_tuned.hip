@@ -127,6 +127,8 @@ struct TunedPlan {
 
 namespace {
 
+constexpr int kLdsMaxSteps = 4;  // LINE_LDS whole-line mode: steps fused per pass
+
 bool params_finite(const bkma_adv_params &p) {
     return std::isfinite(p.dt) && std::isfinite(p.minRealVx) && std::isfinite(p.dvx) && std::isfinite(p.inv_dx) &&
            std::isfinite(p.minRealX) && p.realWidthX > 0 && std::isfinite(p.realWidthX);
@@ -242,8 +244,8 @@ bkma_status stream_feasible(const bkma_adv_params &p, bkma_dtype dt, const Strea
 }
 
 // LINE_LDS shape for a given segment length (0 = whole line).
-bkma_status lds_shape(const bkma_adv_params &p, bkma_dtype dt, i64 seg_rows, i64 chunk_segs, LdsArgs *a, u64 *bytes,
-                      std::string *why) {
+bkma_status lds_shape(const bkma_adv_params &p, bkma_dtype dt, i64 seg_rows, i64 chunk_segs, int ns, LdsArgs *a,
+                      u64 *bytes, std::string *why) {
     if (!shift_supported(p, why)) return BKMA_ERR_UNSUPPORTED;
     u64 lds_max = 0;
     BKMA_TRY(lds_limit_bytes(&lds_max));
@@ -267,7 +269,14 @@ bkma_status lds_shape(const bkma_adv_params &p, bkma_dtype dt, i64 seg_rows, i64
         if (why) *why = "segments shorter than the halo cannot be chained";
         return BKMA_ERR_UNSUPPORTED;
     }
+    a->ns = std::max(1, ns);
+    if (a->ns > 1 && a->nseg > 1) {
+        if (why) *why = "several steps per pass need the whole line in LDS";
+        return BKMA_ERR_UNSUPPORTED;
+    }
     *bytes = (u64)(a->seg_rows + a->hb_max + a->ha_max) * p.n2 * esz;
+    if (a->ns > 1)  // FP64 work buffers of the fused steps (16-byte aligned after the staged line)
+        *bytes = ((*bytes + 15) & ~(u64)15) + (u64)(a->ns > 2 ? 2 : 1) * p.n1 * p.n2 * 8;
     if (*bytes > lds_max) {
         if (why) *why = "segment does not fit in LDS";
         return BKMA_ERR_UNSUPPORTED;
@@ -280,8 +289,9 @@ bkma_status lds_shape(const bkma_adv_params &p, bkma_dtype dt, i64 seg_rows, i64
 }
 
 template <class T>
-bkma_status launch_lds(const TunedPlan &pl, void *data, hipStream_t s) {
-    const LdsArgs &a = pl.lds;
+bkma_status launch_lds(const TunedPlan &pl, void *data, hipStream_t s, int ns) {
+    LdsArgs a = pl.lds;
+    a.ns = ns;
     const unsigned units = (unsigned)std::min<i64>((i64)pl.params.n0 * a.nseg, kLdsMaxGrid);
     if (a.nseg > 1) {
         // halo snapshot per chunk: the chunk's rows seen as one long segment
@@ -309,7 +319,7 @@ bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s, int ns) 
         return bkma_adv_run(BKMA_IMPL_ADAPTIVE_WG, pl.dtype, data, &pl.params, &pl.optim, none, s);
     }
     if (pl.cfg.strategy == BKMA_STRATEGY_LINE_LDS)
-        return pl.dtype == BKMA_F64 ? launch_lds<double>(pl, data, s) : launch_lds<float>(pl, data, s);
+        return pl.dtype == BKMA_F64 ? launch_lds<double>(pl, data, s, ns) : launch_lds<float>(pl, data, s, ns);
     if (ns < 1 || ns > pl.variant->nsmax) return fail(BKMA_ERR_RUNTIME, "tuned advection: steps per pass out of range");
     const StreamVariant &v = *pl.variant;
     StreamArgs a;
@@ -347,7 +357,7 @@ std::string lds_name(bkma_dtype dt, const LdsArgs &a) {
            (a.nseg > 1 ? ",seg" + std::to_string(a.seg_rows) + "x" + std::to_string(a.nseg) + ",chunk" +
                              std::to_string(a.chunk_segs)
                        : ",whole_line") +
-           ">";
+           (a.ns > 1 ? ",steps" + std::to_string(a.ns) : "") + ">";
 }
 
 // Resident work-groups on the whole device for a variant (persistent grid).
@@ -411,10 +421,12 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
         return BKMA_OK;
     }
     if (cfg.strategy == BKMA_STRATEGY_LINE_LDS) {
-        if (steps_per_pass(cfg) != 1)
-            return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: LINE_LDS runs one step per pass");
-        pl->cfg.steps_per_pass = 1;
-        if (lds_shape(p, dt, cfg.seg_rows, cfg.chunk_segs, &pl->lds, &pl->lds_bytes, &why) != BKMA_OK)
+        const int ns = steps_per_pass(cfg);
+        if (ns > kLdsMaxSteps)
+            return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: LINE_LDS fuses at most 4 steps per pass");
+        pl->cfg.steps_per_pass = ns;
+        pl->ns = ns;
+        if (lds_shape(p, dt, cfg.seg_rows, cfg.chunk_segs, ns, &pl->lds, &pl->lds_bytes, &why) != BKMA_OK)
             return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
         if (pl->lds.nseg > 1) {
             const u64 esz = dt == BKMA_F64 ? 8 : 4;
@@ -485,17 +497,20 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
     }
     for (i64 sr : segs) {
         for (i64 cs : {1, 4, 16, 64}) {
-            if (lds_shape(p, dt, sr, cs, &a, &bytes, why) != BKMA_OK) continue;
-            bkma_tuned_config c;
-            std::memset(&c, 0, sizeof(c));
-            c.strategy = BKMA_STRATEGY_LINE_LDS;
-            c.steps_per_pass = 1;
-            c.seg_rows = (int32_t)(a.nseg > 1 ? a.seg_rows : 0);
-            c.chunk_segs = (int32_t)a.chunk_segs;
-            bool dup = false;
-            for (auto &o : out)
-                dup |= o.strategy == c.strategy && o.seg_rows == c.seg_rows && o.chunk_segs == c.chunk_segs;
-            if (!dup) out.push_back(c);
+            for (int ns = 1; ns <= kLdsMaxSteps; ++ns) {
+                if (lds_shape(p, dt, sr, cs, ns, &a, &bytes, why) != BKMA_OK) continue;
+                bkma_tuned_config c;
+                std::memset(&c, 0, sizeof(c));
+                c.strategy = BKMA_STRATEGY_LINE_LDS;
+                c.steps_per_pass = ns;
+                c.seg_rows = (int32_t)(a.nseg > 1 ? a.seg_rows : 0);
+                c.chunk_segs = (int32_t)a.chunk_segs;
+                bool dup = false;
+                for (auto &o : out)
+                    dup |= o.strategy == c.strategy && o.seg_rows == c.seg_rows && o.chunk_segs == c.chunk_segs &&
+                           o.steps_per_pass == c.steps_per_pass;
+                if (!dup) out.push_back(c);
+            }
         }
     }
     // the reference's AdaptiveWg executor and dispatch policy (FP64 storage,
@@ -541,7 +556,7 @@ int heuristic_score(const bkma_tuned_config &c, bkma_dtype dt, const bkma_adv_pa
         return 1000000000 - c.smax * 1000000 - std::abs(ns - want) * 100000 + c.vec * 1000 + c.prefetch * 10 +
                c.nontemporal - c.persistent - 100 * c.wide_window;
     }
-    return c.seg_rows == 0 ? 500000000
+    return c.seg_rows == 0 ? 500000000 - std::abs(steps_per_pass(c) - 2) * 1000
                            : 100000000 + std::min(c.seg_rows, 1 << 24) / 1024 - std::abs(c.chunk_segs - 16) * 10;
 }
 

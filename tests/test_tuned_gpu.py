@@ -313,6 +313,43 @@ def test_line_lds_bit_exact(cuda, case, dtype):
     assert np.array_equal(got, ref), np.abs(got.astype(np.float64) - ref).max()
 
 
+def _lds_config(ns, seg_rows=0):
+    c = B.TunedConfig()
+    c.strategy = B.STRATEGY_LINE_LDS
+    c.seg_rows = seg_rows
+    c.steps_per_pass = ns
+    return c
+
+
+@pytest.mark.parametrize("ns", [2, 3, 4])
+@pytest.mark.parametrize("shape", [(3, 200, 8), (2, 512, 1), (4, 64, 5)])
+@pytest.mark.parametrize("dtype", [B.F64, B.F32])
+def test_line_lds_fused_steps(cuda, dtype, shape, ns):
+    """Whole-line LINE_LDS advancing the staged line ns times in LDS: FP64
+    bit-identical to the reference's steps, FP32 to the restatement with FP64
+    intermediate steps (9 steps: full passes and a shorter last one)."""
+    n0, n1, n2 = shape
+    p = B.make_params(n0, n1, n2, dt=2e-3, minRealVx=-1, maxRealVx=1)
+    po = O.make_params(n0, n1, n2, dt=2e-3, minRealVx=-1, maxRealVx=1)
+    np_dt = np.float64 if dtype == B.F64 else np.float32
+    host = O.uniform(n0 * n1 * n2, 10, np_dt).reshape(shape)
+    got, spp = _run(host, p, dtype, 9, config=_lds_config(ns))
+    assert spp == ns
+    ref = host.copy()
+    if dtype == B.F64:
+        O.advect_f64(ref, po, 9)
+    else:
+        O.advect_f32_blocked(ref, po, 9, ns)
+    bits = np.uint64 if dtype == B.F64 else np.uint32
+    assert np.array_equal(got.view(bits), ref.view(bits))
+
+
+def test_line_lds_fused_steps_need_the_whole_line(cuda):
+    p = B.make_params(2, 4096, 8, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    with pytest.raises(B.BkmaUnsupported):
+        B.AdvPlan(p, B.F32, config=_lds_config(2, seg_rows=512))
+
+
 def test_chained_segments_shorter_than_halo_rejected(cuda):
     # v in {-1, -0.5, 0, 0.5}: backward shifts up to ~31 cells need 33 halo
     # rows before a segment, more than a 32-row segment can carry to the next
