@@ -202,7 +202,8 @@ typedef struct bkma_tuned_config {
     int32_t wide_window;  /* COLUMN_STREAM, F32 storage: 1 = FP64 register window (rows widened once) */
     int32_t steps_per_pass; /* COLUMN_STREAM, COLUMN_PAIR, LINE_LDS whole line (at most 4): time steps fused into
                                one pass over HBM (0 or 1: one) */
-    int32_t chunk_segs;   /* LINE_LDS segmented: consecutive segments one work-group walks in order (0 or 1: one) */
+    int32_t chunk_segs;   /* LINE_LDS segmented: consecutive segments one work-group walks in order (0 or 1: one);
+                             LINE_LDS whole line: consecutive slabs one work-group stages together */
     int32_t dpp_broadcast; /* COLUMN_PAIR: 1 = the coefficients reach the stencil sums through the DPP row
                               broadcast; 0 = through v_readlane */
 } bkma_tuned_config;

@@ -29,7 +29,8 @@ struct LdsArgs {
     i64 ha_max;      // halo rows after a segment (allocation bound)
     i64 chunk_segs;  // segmented mode: consecutive segments one work-group walks in order
     i64 nchunk;      // chunks per slab
-    i64 ns;          // whole-line mode: steps per pass (1: line_lds_unit)
+    i64 ns;          // whole-line mode: steps per pass
+    i64 slabs;       // whole-line mode: consecutive slabs one unit stages together (short lines)
 };
 
 // Halo extents of slab i0: rows [r0 - hb, r0) and [r1, r1 + ha).
@@ -193,13 +194,15 @@ __device__ __forceinline__ void line_lds_unit(T *__restrict__ slab, const T *__r
         lds_compute<T, SEG, 1>(slab, p, i0, r0, nr, hb, lds_rows, lds);
 }
 
-// One step of a whole staged line: dst = step(src), both holding all n1 rows
-// of n2 columns (src in LDS, dst in LDS or the slab in HBM).  Arithmetic of
-// the storage type T: FP64 storage sums with separately rounded multiply and
-// add, FP32 storage with fused ones, in FP64 either way.  Thread (row offset,
-// group of VC columns); the foot is computed once per (row, group).
+// One step of whole staged lines: dst = step(src) for the `ns` consecutive
+// slabs i0 .. i0+ns-1, both holding their n1 rows of n2 columns each (src in
+// LDS, dst in LDS or the slabs in HBM).  Arithmetic of the storage type T:
+// FP64 storage sums with separately rounded multiply and add, FP32 storage
+// with fused ones, in FP64 either way.  Thread (row of some slab, group of VC
+// columns); the foot is computed once per (row, group).
 template <class T, int VC, class S, class D>
-__device__ __forceinline__ void line_step(const S *__restrict__ src, D *__restrict__ dst, const AdvDev &p, i64 i0) {
+__device__ __forceinline__ void line_step(const S *__restrict__ src, D *__restrict__ dst, const AdvDev &p, i64 i0,
+                                          i64 nslab) {
     const i64 n1 = p.n1, n2 = p.n2;
     const int nthr = blockDim.x;
     const i64 nch = n2 / VC;
@@ -209,9 +212,12 @@ __device__ __forceinline__ void line_step(const S *__restrict__ src, D *__restri
     if (t_row >= rows_per_pass) return;
     double coef[6];
     unsigned long long cached = ~0ull;
-    for (i64 i1 = t_row; i1 < n1; i1 += rows_per_pass) {
+    for (i64 rr = t_row; rr < nslab * n1; rr += rows_per_pass) {
+        const i64 g = rr / n1, i1 = rr - g * n1;
+        const S *gsrc = src + g * n1 * n2;
+        D *gdst = dst + g * n1 * n2;
         double d_prev1;
-        const int leftNode = adv_foot(p, (int)i0, (int)i1, d_prev1);
+        const int leftNode = adv_foot(p, (int)(i0 + g), (int)i1, d_prev1);
         const unsigned long long bits = __builtin_bit_cast(unsigned long long, d_prev1);
         if (bits != cached) {
             adv_lag_basis(d_prev1, coef);
@@ -228,7 +234,7 @@ __device__ __forceinline__ void line_step(const S *__restrict__ src, D *__restri
             for (int k = 0; k < 6; ++k) {
 #pragma unroll
                 for (int v = 0; v < VC; ++v) {
-                    const double f = (double)src[r * n2 + ch * VC + v];
+                    const double f = (double)gsrc[r * n2 + ch * VC + v];
                     if constexpr (sizeof(T) == 8) acc[v] = acc[v] + coef[k] * f;
                     else acc[v] = fma(coef[k], f, acc[v]);
                 }
@@ -236,27 +242,27 @@ __device__ __forceinline__ void line_step(const S *__restrict__ src, D *__restri
                 if (r >= n1) r -= n1;
             }
 #pragma unroll
-            for (int v = 0; v < VC; ++v) dst[i1 * n2 + ch * VC + v] = (D)acc[v];
+            for (int v = 0; v < VC; ++v) gdst[i1 * n2 + ch * VC + v] = (D)acc[v];
         }
     }
 }
 
-// Whole line of slab i0 advanced by ns steps in one pass: staged once into
-// `lds` (storage type), steps 1 .. ns-1 into the FP64 buffers w0, w1 in
-// turn, step ns from LDS to the slab in HBM.  Every thread calls it; it has
-// no early return.
+// Whole lines of the nslab consecutive slabs i0 .. advanced by ns steps in
+// one pass: staged once into `lds` (storage type), steps 1 .. ns-1 into the
+// FP64 buffers w0, w1 in turn, step ns from LDS to the slabs in HBM.  Every
+// thread calls it; it has no early return.
 template <class T>
-__device__ __forceinline__ void line_steps_unit(T *__restrict__ slab, const AdvDev &p, i64 i0, int ns,
+__device__ __forceinline__ void line_steps_unit(T *__restrict__ slab, const AdvDev &p, i64 i0, i64 nslab, int ns,
                                                 T *__restrict__ lds, double *__restrict__ w0,
                                                 double *__restrict__ w1) {
-    const i64 cnt = p.n1 * p.n2;
+    const i64 cnt = nslab * p.n1 * p.n2;
     for (i64 e = threadIdx.x; e < cnt; e += blockDim.x) lds[e] = slab[e];
     __syncthreads();
     auto step = [&](const auto *src, auto *dst) __attribute__((always_inline)) {
         if (p.n2 % 2 == 0)
-            line_step<T, 2>(src, dst, p, i0);
+            line_step<T, 2>(src, dst, p, i0, nslab);
         else
-            line_step<T, 1>(src, dst, p, i0);
+            line_step<T, 1>(src, dst, p, i0, nslab);
         __syncthreads();
     };
     if (ns == 1) {
@@ -277,17 +283,26 @@ template <class T, bool SEG>
 __global__ __launch_bounds__(256) void adv_line_lds_kernel(T *__restrict__ data, const T *__restrict__ halo,
                                                            LdsArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
+    if (!SEG && (a.ns > 1 || a.slabs > 1)) {
+        // whole lines of `slabs` slabs, ns steps: [slabs][n1][n2] T, then two
+        // such FP64 buffers
+        const i64 per = a.p.n1 * a.p.n2, cnt = a.slabs * per;
+        T *st = reinterpret_cast<T *>(lds_raw);
+        double *w0 = reinterpret_cast<double *>(lds_raw + ((cnt * (i64)sizeof(T) + 15) & ~(i64)15));
+        const i64 units = (a.p.n0 + a.slabs - 1) / a.slabs;
+        for (i64 unit = blockIdx.x; unit < units; unit += gridDim.x) {
+            const i64 i0 = unit * a.slabs;
+            const i64 nslab = min(a.slabs, a.p.n0 - i0);
+            line_steps_unit<T>(data + i0 * per, a.p, i0, nslab, (int)a.ns, st, w0, w0 + cnt);
+            __syncthreads();  // the next unit restages the LDS
+        }
+        return;
+    }
     for (i64 unit = blockIdx.x; unit < a.p.n0 * a.nseg; unit += gridDim.x) {
         const i64 i0 = unit / a.nseg, seg = unit - i0 * a.nseg;
         const i64 r0 = seg * a.seg_rows;
         const i64 r1 = min(a.p.n1, r0 + a.seg_rows);
-        if (!SEG && a.ns > 1) {
-            // whole line, ns steps: [n1*n2] T, then two [n1*n2] FP64 buffers
-            const i64 cnt = a.p.n1 * a.p.n2;
-            T *st = reinterpret_cast<T *>(lds_raw);
-            double *w0 = reinterpret_cast<double *>(lds_raw + ((cnt * (i64)sizeof(T) + 15) & ~(i64)15));
-            line_steps_unit<T>(data + i0 * cnt, a.p, i0, (int)a.ns, st, w0, w0 + cnt);
-        } else {
+        {
             line_lds_unit<T, SEG>(data + i0 * a.p.n1 * a.p.n2,
                                   SEG ? halo + unit * (a.hb_max + a.ha_max) * a.p.n2 : nullptr, a.p, i0, r0, r1,
                                   a.hb_max, a.ha_max, reinterpret_cast<T *>(lds_raw));

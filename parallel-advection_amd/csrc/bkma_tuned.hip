@@ -264,6 +264,9 @@ bkma_status lds_shape(const bkma_adv_params &p, bkma_dtype dt, i64 seg_rows, i64
         a->ha_max = std::max(0, 3 - dlo);
     }
     a->chunk_segs = a->nseg > 1 ? std::max<i64>(1, std::min<i64>(chunk_segs, a->nseg)) : 1;
+    // whole lines: chunk_segs is the number of consecutive slabs a unit
+    // stages together (short lines)
+    a->slabs = a->nseg == 1 ? std::max<i64>(1, std::min<i64>(chunk_segs, (i64)p.n0)) : 1;
     a->nchunk = (i64)ceil_div((u64)a->nseg, (u64)a->chunk_segs);
     if (a->chunk_segs > 1 && a->seg_rows < a->hb_max) {
         if (why) *why = "segments shorter than the halo cannot be chained";
@@ -274,9 +277,13 @@ bkma_status lds_shape(const bkma_adv_params &p, bkma_dtype dt, i64 seg_rows, i64
         if (why) *why = "several steps per pass need the whole line in LDS";
         return BKMA_ERR_UNSUPPORTED;
     }
-    *bytes = (u64)(a->seg_rows + a->hb_max + a->ha_max) * p.n2 * esz;
-    if (a->ns > 1)  // FP64 work buffers of the fused steps (16-byte aligned after the staged line)
-        *bytes = ((*bytes + 15) & ~(u64)15) + (u64)(a->ns > 2 ? 2 : 1) * p.n1 * p.n2 * 8;
+    auto need = [&](i64 slabs) {
+        u64 b = (u64)slabs * (a->seg_rows + a->hb_max + a->ha_max) * p.n2 * esz;
+        if (a->ns > 1)  // FP64 work buffers of the fused steps (16-byte aligned after the staged lines)
+            b = ((b + 15) & ~(u64)15) + (u64)(a->ns > 2 ? 2 : 1) * slabs * p.n1 * p.n2 * 8;
+        return b;
+    };
+    *bytes = need(a->slabs);
     if (*bytes > lds_max) {
         if (why) *why = "segment does not fit in LDS";
         return BKMA_ERR_UNSUPPORTED;
@@ -292,7 +299,8 @@ template <class T>
 bkma_status launch_lds(const TunedPlan &pl, void *data, hipStream_t s, int ns) {
     LdsArgs a = pl.lds;
     a.ns = ns;
-    const unsigned units = (unsigned)std::min<i64>((i64)pl.params.n0 * a.nseg, kLdsMaxGrid);
+    const i64 nunits = a.slabs > 1 ? ((i64)pl.params.n0 + a.slabs - 1) / a.slabs : (i64)pl.params.n0 * a.nseg;
+    const unsigned units = (unsigned)std::min<i64>(nunits, kLdsMaxGrid);
     if (a.nseg > 1) {
         // halo snapshot per chunk: the chunk's rows seen as one long segment
         LdsArgs sa = a;
@@ -357,7 +365,8 @@ std::string lds_name(bkma_dtype dt, const LdsArgs &a) {
            (a.nseg > 1 ? ",seg" + std::to_string(a.seg_rows) + "x" + std::to_string(a.nseg) + ",chunk" +
                              std::to_string(a.chunk_segs)
                        : ",whole_line") +
-           (a.ns > 1 ? ",steps" + std::to_string(a.ns) : "") + ">";
+           (a.slabs > 1 ? ",slabs" + std::to_string(a.slabs) : "") + (a.ns > 1 ? ",steps" + std::to_string(a.ns) : "") +
+           ">";
 }
 
 // Resident work-groups on the whole device for a variant (persistent grid).
@@ -437,7 +446,7 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
             }
         }
         pl->cfg.seg_rows = pl->lds.nseg > 1 ? (int32_t)pl->lds.seg_rows : 0;
-        pl->cfg.chunk_segs = (int32_t)pl->lds.chunk_segs;
+        pl->cfg.chunk_segs = (int32_t)(pl->lds.nseg > 1 ? pl->lds.chunk_segs : pl->lds.slabs);
         pl->name = lds_name(dt, pl->lds);
         return BKMA_OK;
     }
@@ -496,7 +505,9 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
         if (rows >= 32 && (u64)rows < p.n1) segs.push_back(rows & ~(i64)3);
     }
     for (i64 sr : segs) {
-        for (i64 cs : {1, 4, 16, 64}) {
+        std::vector<i64> css = {1, 4, 16, 64};
+        if (sr == 0 && p.n1 * p.n2 < 4096) css.push_back((i64)(4096 / (p.n1 * p.n2)));  // slab groups
+        for (i64 cs : css) {
             for (int ns = 1; ns <= kLdsMaxSteps; ++ns) {
                 if (lds_shape(p, dt, sr, cs, ns, &a, &bytes, why) != BKMA_OK) continue;
                 bkma_tuned_config c;
@@ -504,7 +515,7 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
                 c.strategy = BKMA_STRATEGY_LINE_LDS;
                 c.steps_per_pass = ns;
                 c.seg_rows = (int32_t)(a.nseg > 1 ? a.seg_rows : 0);
-                c.chunk_segs = (int32_t)a.chunk_segs;
+                c.chunk_segs = (int32_t)(a.nseg > 1 ? a.chunk_segs : a.slabs);
                 bool dup = false;
                 for (auto &o : out)
                     dup |= o.strategy == c.strategy && o.seg_rows == c.seg_rows && o.chunk_segs == c.chunk_segs &&

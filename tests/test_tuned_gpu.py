@@ -313,27 +313,31 @@ def test_line_lds_bit_exact(cuda, case, dtype):
     assert np.array_equal(got, ref), np.abs(got.astype(np.float64) - ref).max()
 
 
-def _lds_config(ns, seg_rows=0):
+def _lds_config(ns, seg_rows=0, slabs=1):
     c = B.TunedConfig()
     c.strategy = B.STRATEGY_LINE_LDS
     c.seg_rows = seg_rows
     c.steps_per_pass = ns
+    c.chunk_segs = slabs  # whole lines: slabs staged together
     return c
 
 
-@pytest.mark.parametrize("ns", [2, 3, 4])
-@pytest.mark.parametrize("shape", [(3, 200, 8), (2, 512, 1), (4, 64, 5)])
+@pytest.mark.parametrize("slabs", [1, 7])
+@pytest.mark.parametrize("ns", [1, 2, 3, 4])
+@pytest.mark.parametrize("shape", [(3, 200, 8), (2, 512, 1), (4, 64, 5), (300, 16, 1)])
 @pytest.mark.parametrize("dtype", [B.F64, B.F32])
-def test_line_lds_fused_steps(cuda, dtype, shape, ns):
-    """Whole-line LINE_LDS advancing the staged line ns times in LDS: FP64
+def test_line_lds_fused_steps(cuda, dtype, shape, ns, slabs):
+    """Whole-line LINE_LDS advancing the staged lines ns times in LDS: FP64
     bit-identical to the reference's steps, FP32 to the restatement with FP64
-    intermediate steps (9 steps: full passes and a shorter last one)."""
+    intermediate steps (9 steps: full passes and a shorter last one).  With
+    slabs = 7, units stage up to 7 consecutive slabs (a partial last unit for
+    n0 = 4 and 300)."""
     n0, n1, n2 = shape
     p = B.make_params(n0, n1, n2, dt=2e-3, minRealVx=-1, maxRealVx=1)
     po = O.make_params(n0, n1, n2, dt=2e-3, minRealVx=-1, maxRealVx=1)
     np_dt = np.float64 if dtype == B.F64 else np.float32
     host = O.uniform(n0 * n1 * n2, 10, np_dt).reshape(shape)
-    got, spp = _run(host, p, dtype, 9, config=_lds_config(ns))
+    got, spp = _run(host, p, dtype, 9, config=_lds_config(ns, slabs=slabs))
     assert spp == ns
     ref = host.copy()
     if dtype == B.F64:
