@@ -43,8 +43,8 @@ __device__ __forceinline__ void slab_halo(const AdvDev &p, i64 i0, i64 hb_max, i
 }
 
 // Copies, for every (slab, segment), the old halo rows into
-// halo[(i0 * nseg + seg) * (hb_max + ha_max) * n2 ...]: first the hb rows
-// before the segment, then the ha rows after it (periodic in i1).
+// halo[(i0 * nseg + seg) * ns * (hb_max + ha_max) * n2 ...]: first the ns*hb
+// rows before the segment, then the ns*ha rows after it (periodic in i1).
 template <class T>
 __device__ __forceinline__ void halo_snapshot_unit(const T *__restrict__ data, T *__restrict__ halo,
                                                    const LdsArgs &a, i64 unit) {
@@ -55,8 +55,10 @@ __device__ __forceinline__ void halo_snapshot_unit(const T *__restrict__ data, T
     const i64 r1 = min(n1, r0 + a.seg_rows);
     int hb, ha;
     slab_halo(p, i0, a.hb_max, a.ha_max, hb, ha);
+    hb *= (int)a.ns;  // ns fused steps reach ns times as far
+    ha *= (int)a.ns;
     const T *slab = data + i0 * n1 * n2;
-    T *dst = halo + unit * (a.hb_max + a.ha_max) * n2;
+    T *dst = halo + unit * a.ns * (a.hb_max + a.ha_max) * n2;
     const i64 total = (i64)(hb + ha) * n2;
     for (i64 e = threadIdx.x; e < total; e += blockDim.x) {
         const i64 hr = e / n2, c = e - hr * n2;
@@ -366,17 +368,135 @@ __device__ __forceinline__ void line_chunk_unit(T *__restrict__ slab, const T *_
     }
 }
 
+// One step over rows [lo, hi) of slab i0 (unwrapped row numbers: rows
+// before 0 and from n1 are the line's periodic continuation), reading src
+// and writing dst, both indexed by row - base (LDS buffers of the staged
+// rows) -- or, for a dst in HBM (TOHBM), by the row itself.  The stencil of
+// row r lies delta = ipos1 - r rows from it (normalised into (-n1/2, n1/2]),
+// the foot being that of row r mod n1.  Arithmetic of the storage type T,
+// in FP64.  Thread (row, group of VC columns).
+template <class T, int VC, bool TOHBM, class S, class D>
+__device__ __forceinline__ void seg_step(const S *__restrict__ src, D *__restrict__ dst, const AdvDev &p, i64 i0,
+                                         i64 lo, i64 hi, i64 base) {
+    const i64 n1 = p.n1, n2 = p.n2;
+    const int nthr = blockDim.x;
+    const i64 nch = n2 / VC;
+    const int cw = (int)min<i64>(nch, nthr);
+    const int rows_per_pass = nthr / cw;
+    const int t_row = threadIdx.x / cw, t_col = threadIdx.x - t_row * cw;
+    if (t_row >= rows_per_pass) return;
+    const int half = (int)(n1 / 2);
+    double coef[6];
+    unsigned long long cached = ~0ull;
+    for (i64 r = lo + t_row; r < hi; r += rows_per_pass) {
+        i64 rw = r % n1;
+        if (rw < 0) rw += n1;
+        double d_prev1;
+        const int leftNode = adv_foot(p, (int)i0, (int)rw, d_prev1);
+        const unsigned long long bits = __builtin_bit_cast(unsigned long long, d_prev1);
+        if (bits != cached) {
+            adv_lag_basis(d_prev1, coef);
+            cached = bits;
+        }
+        int delta = leftNode - 2 - (int)rw;
+        if (delta < -half) delta += (int)n1;
+        if (delta > half) delta -= (int)n1;
+        const S *row0 = src + (r - base + delta) * n2;
+        for (i64 ch = t_col; ch < nch; ch += cw) {
+            double acc[VC];
+#pragma unroll
+            for (int v = 0; v < VC; ++v) acc[v] = 0.;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+#pragma unroll
+                for (int v = 0; v < VC; ++v) {
+                    const double f = (double)row0[k * n2 + ch * VC + v];
+                    if constexpr (sizeof(T) == 8) acc[v] = acc[v] + coef[k] * f;
+                    else acc[v] = fma(coef[k], f, acc[v]);
+                }
+            }
+            D *out = dst + (TOHBM ? r : r - base) * n2 + ch * VC;
+#pragma unroll
+            for (int v = 0; v < VC; ++v) out[v] = (D)acc[v];
+        }
+    }
+}
+
+// line_chunk_unit advancing ns steps per pass.  A segment stages its rows
+// with ns*hb old rows before and ns*ha after (carried from the previous
+// segment, read from HBM, or from the chunk's snapshot, as line_chunk_unit
+// does with one step's halo); step t computes the rows that steps t+1 .. ns
+// still need, [r0 - (ns-t)*hb, r1 + (ns-t)*ha), into the FP64 buffers w0 and
+// w1 in turn; step ns writes [r0, r1) to HBM.  Needs seg_rows >= ns*hb_max.
+template <class T>
+__device__ __forceinline__ void line_chunk_steps_unit(T *__restrict__ slab, const T *__restrict__ hsnap,
+                                                      const AdvDev &p, i64 i0, i64 c0, i64 c1, i64 seg_rows,
+                                                      i64 hb_max, i64 ha_max, int ns, T *__restrict__ st,
+                                                      double *__restrict__ w0, double *__restrict__ w1) {
+    const i64 n2 = p.n2;
+    const int nthr = blockDim.x;
+    int hb, ha;
+    slab_halo(p, i0, hb_max, ha_max, hb, ha);
+    const i64 HB = (i64)ns * hb, HA = (i64)ns * ha;
+    for (i64 r0 = c0; r0 < c1; r0 += seg_rows) {
+        const i64 r1 = min(c1, r0 + seg_rows);
+        const i64 nr = r1 - r0;
+        const i64 base = r0 - HB;  // global row of buffer row 0
+        if (r0 == c0)
+            for (i64 e = threadIdx.x; e < HB * n2; e += nthr) st[e] = hsnap[e];
+        for (i64 e = threadIdx.x; e < nr * n2; e += nthr) st[HB * n2 + e] = slab[r0 * n2 + e];
+        for (i64 e = threadIdx.x; e < HA * n2; e += nthr) {
+            const i64 g = r1 + e / n2;  // global row
+            st[(HB + nr) * n2 + e] = g < c1 ? slab[r1 * n2 + e] : hsnap[HB * n2 + (g - c1) * n2 + e % n2];
+        }
+        __syncthreads();
+        auto step = [&](const auto *src, auto *dst, i64 lo, i64 hi, auto tohbm_) __attribute__((always_inline)) {
+            constexpr bool TOHBM = decltype(tohbm_)::value;
+            if (n2 % 4 == 0)
+                seg_step<T, 4, TOHBM>(src, dst, p, i0, lo, hi, base);
+            else if (n2 % 2 == 0)
+                seg_step<T, 2, TOHBM>(src, dst, p, i0, lo, hi, base);
+            else
+                seg_step<T, 1, TOHBM>(src, dst, p, i0, lo, hi, base);
+            __syncthreads();
+        };
+        const T *s0 = st;
+        step(s0, w0, r0 - (ns - 1) * hb, r1 + (ns - 1) * ha, std::false_type{});
+        for (int t = 2; t < ns; ++t) {
+            const i64 lo = r0 - (i64)(ns - t) * hb, hi = r1 + (i64)(ns - t) * ha;
+            if (t % 2 == 0)
+                step((const double *)w0, w1, lo, hi, std::false_type{});
+            else
+                step((const double *)w1, w0, lo, hi, std::false_type{});
+        }
+        step((const double *)(ns % 2 == 0 ? w0 : w1), slab, r0, r1, std::true_type{});
+        if (r1 < c1) {  // old rows [r1 - HB, r1) move to the front for the next segment
+            for (i64 e = threadIdx.x; e < HB * n2; e += nthr) st[e] = st[nr * n2 + e];
+            __syncthreads();
+        }
+    }
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void adv_line_chunk_kernel(T *__restrict__ data, const T *__restrict__ halo,
                                                              LdsArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
+    // fused steps: the staged rows (storage type), then two FP64 buffers of as many rows
+    const i64 rows = a.ns * (a.hb_max + a.ha_max) + a.seg_rows;
+    T *st = reinterpret_cast<T *>(lds_raw);
+    double *w0 = reinterpret_cast<double *>(lds_raw + ((rows * a.p.n2 * (i64)sizeof(T) + 15) & ~(i64)15));
     for (i64 unit = blockIdx.x; unit < a.p.n0 * a.nchunk; unit += gridDim.x) {
         const i64 i0 = unit / a.nchunk, ch = unit - i0 * a.nchunk;
         const i64 chunk_rows = a.seg_rows * a.chunk_segs;
         const i64 c0 = ch * chunk_rows;
         const i64 c1 = min(a.p.n1, c0 + chunk_rows);
-        line_chunk_unit<T>(data + i0 * a.p.n1 * a.p.n2, halo + unit * (a.hb_max + a.ha_max) * a.p.n2, a.p, i0, c0,
-                           c1, a.seg_rows, a.hb_max, a.ha_max, reinterpret_cast<T *>(lds_raw));
+        const T *hs = halo + unit * a.ns * (a.hb_max + a.ha_max) * a.p.n2;
+        if (a.ns > 1)
+            line_chunk_steps_unit<T>(data + i0 * a.p.n1 * a.p.n2, hs, a.p, i0, c0, c1, a.seg_rows, a.hb_max,
+                                     a.ha_max, (int)a.ns, st, w0, w0 + rows * a.p.n2);
+        else
+            line_chunk_unit<T>(data + i0 * a.p.n1 * a.p.n2, hs, a.p, i0, c0, c1, a.seg_rows, a.hb_max, a.ha_max,
+                               reinterpret_cast<T *>(lds_raw));
         __syncthreads();  // the next unit restages the LDS
     }
 }

@@ -127,7 +127,7 @@ struct TunedPlan {
 
 namespace {
 
-constexpr int kLdsMaxSteps = 4;  // LINE_LDS whole-line mode: steps fused per pass
+constexpr int kLdsMaxSteps = 4;  // LINE_LDS: steps fused per pass
 
 bool params_finite(const bkma_adv_params &p) {
     return std::isfinite(p.dt) && std::isfinite(p.minRealVx) && std::isfinite(p.dvx) && std::isfinite(p.inv_dx) &&
@@ -268,19 +268,17 @@ bkma_status lds_shape(const bkma_adv_params &p, bkma_dtype dt, i64 seg_rows, i64
     // stages together (short lines)
     a->slabs = a->nseg == 1 ? std::max<i64>(1, std::min<i64>(chunk_segs, (i64)p.n0)) : 1;
     a->nchunk = (i64)ceil_div((u64)a->nseg, (u64)a->chunk_segs);
-    if (a->chunk_segs > 1 && a->seg_rows < a->hb_max) {
+    a->ns = std::max(1, ns);
+    if (a->chunk_segs > 1 && a->seg_rows < a->ns * a->hb_max) {
         if (why) *why = "segments shorter than the halo cannot be chained";
         return BKMA_ERR_UNSUPPORTED;
     }
-    a->ns = std::max(1, ns);
-    if (a->ns > 1 && a->nseg > 1) {
-        if (why) *why = "several steps per pass need the whole line in LDS";
-        return BKMA_ERR_UNSUPPORTED;
-    }
     auto need = [&](i64 slabs) {
-        u64 b = (u64)slabs * (a->seg_rows + a->hb_max + a->ha_max) * p.n2 * esz;
-        if (a->ns > 1)  // FP64 work buffers of the fused steps (16-byte aligned after the staged lines)
-            b = ((b + 15) & ~(u64)15) + (u64)(a->ns > 2 ? 2 : 1) * slabs * p.n1 * p.n2 * 8;
+        // staged rows: the segment (or whole lines) and ns steps' halos
+        const u64 rows = (u64)(a->seg_rows + a->ns * (a->hb_max + a->ha_max));
+        u64 b = (u64)slabs * rows * p.n2 * esz;
+        if (a->ns > 1)  // FP64 work buffers of the fused steps, 16-byte aligned after the staged rows
+            b = ((b + 15) & ~(u64)15) + (a->nseg > 1 ? 2 : (u64)(a->ns > 2 ? 2 : 1)) * slabs * rows * p.n2 * 8;
         return b;
     };
     *bytes = need(a->slabs);
@@ -439,7 +437,8 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
             return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
         if (pl->lds.nseg > 1) {
             const u64 esz = dt == BKMA_F64 ? 8 : 4;
-            const u64 bytes = p.n0 * (u64)pl->lds.nchunk * (u64)(pl->lds.hb_max + pl->lds.ha_max) * p.n2 * esz;
+            const u64 bytes =
+                p.n0 * (u64)pl->lds.nchunk * (u64)pl->lds.ns * (u64)(pl->lds.hb_max + pl->lds.ha_max) * p.n2 * esz;
             if (hipMalloc(&pl->halo, std::max<u64>(bytes, 16)) != hipSuccess) {
                 pl->halo = nullptr;
                 return fail(BKMA_ERR_OUT_OF_MEMORY, "tuned advection: halo buffer allocation failed");
@@ -496,19 +495,22 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
     const u64 esz = dt == BKMA_F64 ? 8 : 4;
     LdsArgs a;
     u64 bytes;
-    std::vector<i64> segs = {0};
     int dlo = 0, dhi = 0;
     if (params_finite(p)) shift_range(p, &dlo, &dhi);
     const i64 halo_rows = std::max(0, dhi + 2) + std::max(0, 3 - dlo);
-    for (u64 budget : {32768ull, 65536ull, 163840ull}) {
-        const i64 rows = (i64)(budget / std::max<u64>(p.n2 * esz, 1)) - halo_rows;
-        if (rows >= 32 && (u64)rows < p.n1) segs.push_back(rows & ~(i64)3);
-    }
-    for (i64 sr : segs) {
-        std::vector<i64> css = {1, 4, 16, 64};
-        if (sr == 0 && p.n1 * p.n2 < 4096) css.push_back((i64)(4096 / (p.n1 * p.n2)));  // slab groups
-        for (i64 cs : css) {
-            for (int ns = 1; ns <= kLdsMaxSteps; ++ns) {
+    for (int ns = 1; ns <= kLdsMaxSteps; ++ns) {
+        // whole lines, and segments sized for 32/64/160 KB of LDS: staged rows
+        // in the storage type plus, for fused steps, two FP64 work buffers
+        const u64 row_bytes = std::max<u64>(p.n2 * (esz + (ns > 1 ? 16 : 0)), 1);
+        std::vector<i64> segs = {0};
+        for (u64 budget : {32768ull, 65536ull, 163840ull}) {
+            const i64 rows = (i64)(budget / row_bytes) - ns * halo_rows - 1;
+            if (rows >= 32 && (u64)rows < p.n1) segs.push_back(rows & ~(i64)3);
+        }
+        for (i64 sr : segs) {
+            std::vector<i64> css = {1, 4, 16, 64};
+            if (sr == 0 && p.n1 * p.n2 < 4096) css.push_back((i64)(4096 / (p.n1 * p.n2)));  // slab groups
+            for (i64 cs : css) {
                 if (lds_shape(p, dt, sr, cs, ns, &a, &bytes, why) != BKMA_OK) continue;
                 bkma_tuned_config c;
                 std::memset(&c, 0, sizeof(c));

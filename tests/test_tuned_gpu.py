@@ -348,10 +348,55 @@ def test_line_lds_fused_steps(cuda, dtype, shape, ns, slabs):
     assert np.array_equal(got.view(bits), ref.view(bits))
 
 
-def test_line_lds_fused_steps_need_the_whole_line(cuda):
-    p = B.make_params(2, 4096, 8, dt=1e-3, minRealVx=-1, maxRealVx=1)
+FUSED_SEG_CASES = [
+    # (n0, n1, n2, dt, seg_rows, chunk_segs)
+    (3, 4096, 8, 2e-3, 500, 1),      # shift ~8 cells, segments with their own snapshots
+    (3, 4096, 8, 2e-3, 500, 3),      # chained segments (carried halo), ragged last chunk
+    (2, 16384, 4, 1e-3, 1024, 4),    # shift ~16 cells, 4 chunks of 4 segments
+    (2, 16384, 4, 1e-3, 1024, 16),   # one chunk per slab: halos only at the wrap
+    (4, 2048, 1, 1e-2, 256, 3),      # shift ~20 cells, n2 = 1 (scalar columns)
+    (3, 2048, 6, 1e-2, 512, 2),      # n2 = 6 (pairs of columns)
+]
+
+
+def _lds_seg_config(ns, seg_rows, chunk_segs):
+    c = _lds_config(ns, seg_rows=seg_rows)
+    c.chunk_segs = chunk_segs
+    return c
+
+
+@pytest.mark.parametrize("ns", [2, 3, 4])
+@pytest.mark.parametrize("case", FUSED_SEG_CASES,
+                         ids=lambda c: "x".join(map(str, c[:3])) + f"-dt{c[3]}-seg{c[4]}-chunk{c[5]}")
+@pytest.mark.parametrize("dtype", [B.F64, B.F32], ids=["f64", "f32"])
+def test_line_lds_segments_fused_steps(cuda, dtype, case, ns):
+    """Segmented LINE_LDS fusing ns steps: each segment stages ns steps'
+    halos (carried, read ahead, or from the chunk's snapshot) and computes
+    the rows later steps still need (9 steps: full passes and a shorter
+    last one)."""
+    n0, n1, n2, dt, seg, chunk = case
+    p = B.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
+    po = O.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
+    np_dt = np.float64 if dtype == B.F64 else np.float32
+    host = O.uniform(n0 * n1 * n2, 11, np_dt).reshape(n0, n1, n2)
+    got, spp = _run(host, p, dtype, 9, config=_lds_seg_config(ns, seg, chunk))
+    assert spp == ns
+    ref = host.copy()
+    if dtype == B.F64:
+        O.advect_f64(ref, po, 9)
+    else:
+        O.advect_f32_blocked(ref, po, 9, ns)
+    bits = np.uint64 if dtype == B.F64 else np.uint32
+    assert np.array_equal(got.view(bits), ref.view(bits)), np.abs(got.astype(np.float64) - ref).max()
+
+
+def test_line_lds_chained_fused_steps_need_segments_past_the_halo(cuda):
+    # v in {-1, -0.5, 0, 0.5}: backward shifts up to ~21 cells need 23 halo
+    # rows a step; 4 steps reach 92 rows, beyond 64-row segments, 2 steps fit
+    p = B.make_params(4, 2048, 1, dt=2e-2, minRealVx=-1, maxRealVx=1)
     with pytest.raises(B.BkmaUnsupported):
-        B.AdvPlan(p, B.F32, config=_lds_config(2, seg_rows=512))
+        B.AdvPlan(p, B.F32, config=_lds_seg_config(4, 64, 4))
+    B.AdvPlan(p, B.F32, config=_lds_seg_config(2, 64, 4)).close()
 
 
 def test_chained_segments_shorter_than_halo_rejected(cuda):
