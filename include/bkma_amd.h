@@ -186,8 +186,13 @@ typedef enum bkma_strategy {
     BKMA_STRATEGY_LINE_LDS = 2,      /* any n2 and shift: slab segments + halos staged in LDS */
     BKMA_STRATEGY_COLUMN_PAIR = 3,   /* as COLUMN_STREAM, for slabs whose rows take at most two adjacent shifts
                                         (checked per slab at plan creation): seven-row windows, no shift branches */
-    BKMA_STRATEGY_PLAIN = 4          /* the reference's AdaptiveWg executor and create_optim_params dispatch
+    BKMA_STRATEGY_PLAIN = 4,         /* the reference's AdaptiveWg executor and create_optim_params dispatch
                                         (F64 storage, line in LDS; seg_rows = pref_wg_size), one step per pass */
+    BKMA_STRATEGY_HYBRID = 5         /* the reference's local/global split (HybridMem.cpp:4-32, percent_loc of
+                                        AdvectionParams.hpp:18): the first floor(n0 * chunk_segs / 100) slabs run
+                                        the AdaptiveWg executor with the line scratch in LDS, the other slabs with
+                                        a global scratch, concurrently on a second stream (F64 storage;
+                                        seg_rows = pref_wg_size), one step per pass */
 } bkma_strategy;
 
 typedef struct bkma_tuned_config {
@@ -198,12 +203,13 @@ typedef struct bkma_tuned_config {
     int32_t waves_per_wg; /* wavefronts per work-group */
     int32_t nontemporal;  /* 1: non-temporal global loads/stores */
     int32_t persistent;   /* 1: grid capped at the resident work-groups, waves loop over units */
-    int32_t seg_rows;     /* LINE_LDS: rows per segment (0 = whole line in LDS); PLAIN: pref_wg_size */
+    int32_t seg_rows;     /* LINE_LDS: rows per segment (0 = whole line in LDS); PLAIN, HYBRID: pref_wg_size */
     int32_t wide_window;  /* COLUMN_STREAM, F32 storage: 1 = FP64 register window (rows widened once) */
     int32_t steps_per_pass; /* COLUMN_STREAM, COLUMN_PAIR, LINE_LDS whole line (at most 4): time steps fused into
                                one pass over HBM (0 or 1: one) */
     int32_t chunk_segs;   /* LINE_LDS segmented: consecutive segments one work-group walks in order (0 or 1: one);
-                             LINE_LDS whole line: consecutive slabs one work-group stages together */
+                             LINE_LDS whole line: consecutive slabs one work-group stages together;
+                             HYBRID: percent of the slabs whose line scratch is in LDS (0..100) */
     int32_t dpp_broadcast; /* COLUMN_PAIR: 1 = the coefficients reach the stencil sums through the DPP row
                               broadcast; 0 = through v_readlane */
 } bkma_tuned_config;

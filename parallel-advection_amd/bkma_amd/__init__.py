@@ -107,6 +107,7 @@ STRATEGY_COLUMN_STREAM = 1
 STRATEGY_LINE_LDS = 2
 STRATEGY_COLUMN_PAIR = 3
 STRATEGY_PLAIN = 4
+STRATEGY_HYBRID = 5
 
 
 class AdvProblem(ctypes.Structure):

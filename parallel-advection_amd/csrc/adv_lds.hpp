@@ -158,7 +158,7 @@ __device__ __forceinline__ void lds_compute(T *__restrict__ slab, const AdvDev &
 template <class T, bool SEG>
 __device__ __forceinline__ void line_lds_unit(T *__restrict__ slab, const T *__restrict__ hsrc, const AdvDev &p, i64 i0,
                                               i64 r0, i64 r1, i64 hb_max, i64 ha_max, T *__restrict__ lds) {
-    const i64 n1 = p.n1, n2 = p.n2;
+    const i64 n2 = p.n2;
     const i64 nr = r1 - r0;
     const int nthr = blockDim.x;
     int hb = 0, ha = 0;

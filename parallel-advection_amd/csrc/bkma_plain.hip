@@ -31,6 +31,7 @@ struct AwgLaunch {
     i64 w0, w1, w2, g0, g2;
     i64 trips0, trips2;  // uniform per-work-group line iterations
     i64 ge1, ge2;        // global scratch extents 1 and 2 (Global only)
+    i64 i0_base;         // slab index of the launch's first slab (data and the solver's i0)
 };
 
 template <class T, class Solver, bool GLOBAL_SCRATCH>
@@ -62,11 +63,12 @@ __global__ void adaptivewg_kernel(T *__restrict__ data, Solver solver, AwgLaunch
         for (i64 t2 = 0; t2 < L.trips2; ++t2) {
             const i64 i2 = start2 + t2 * L.g2 * L.w2;
             const bool active = (i0 < stop0) && (i2 < stop2);
-            LineRef<T> line{data + (i0 * L.n1) * L.n2 + i2, L.n2};
+            const i64 gi0 = L.i0_base + i0;
+            LineRef<T> line{data + (gi0 * L.n1) * L.n2 + i2, L.n2};
             if (active) {
                 for (i64 ii1 = l1; ii1 < L.n1; ii1 += L.w1) {
                     const i64 iw = ii1 - (window - 1);
-                    if (iw >= 0) scr[iw] = from_f64<T>(solver(line, i0, ii1, i2));
+                    if (iw >= 0) scr[iw] = from_f64<T>(solver(line, gi0, ii1, i2));
                 }
             }
             __syncthreads();
@@ -148,7 +150,7 @@ bkma_status adaptivewg_shape(const bkma_optim_params &o, i64 b0_size, i64 b2_siz
 
 template <class T, class Solver>
 bkma_status run_adaptivewg(T *data, const Solver &solver, i64 n0, i64 n1, i64 n2, const bkma_optim_params &o,
-                           const bkma_span3d &scratch, hipStream_t stream) {
+                           const bkma_span3d &scratch, hipStream_t stream, i64 i0_base = 0) {
     const i64 window = solver.window();
     if (window < 1 || window > n1) return fail(BKMA_ERR_INVALID_ARGUMENT, "solver window larger than n1");
     const i64 nw = n1 - (window - 1);
@@ -186,6 +188,7 @@ bkma_status run_adaptivewg(T *data, const Solver &solver, i64 n0, i64 n1, i64 n2
             L.trips0 = (i64)ceil_div((u64)bs0, (u64)(g0 * w0));
             L.trips2 = (i64)ceil_div((u64)bs2, (u64)(g2 * w2));
             L.ge1 = 0; L.ge2 = 0;
+            L.i0_base = i0_base;
 
             const dim3 block((unsigned)w2, (unsigned)w1, (unsigned)w0);
             const u64 nblocks = (u64)g0 * (u64)g2;
@@ -280,6 +283,15 @@ bkma_status run_impl(bkma_impl impl, T *data, const Solver &solver, i64 n0, i64 
 }
 
 }  // namespace
+
+bkma_status adaptivewg_slabs(double *data, const bkma_adv_params &p, u64 i0_begin, u64 count,
+                             const bkma_optim_params &o, const bkma_span3d &scratch, hipStream_t stream) {
+    if (count == 0 || i0_begin + count > p.n0) return fail(BKMA_ERR_INVALID_ARGUMENT, "AdaptiveWg: slab range out of the array");
+    if (p.n1 > 0x7fffffffULL || p.n0 > 0x7fffffffULL)
+        return fail(BKMA_ERR_UNSUPPORTED, "n0 and n1 must fit in a 32-bit int (the solver's index type)");
+    AdvSolverDev solver{make_adv_dev(p)};
+    return run_adaptivewg(data, solver, (i64)count, (i64)p.n1, (i64)p.n2, o, scratch, stream, (i64)i0_begin);
+}
 
 }  // namespace bkma
 

@@ -113,7 +113,15 @@ struct TunedPlan {
     int ns = 1;                        // steps per pass
     i64 persistent_blocks[9] = {};    // by steps per pass: 0 = one wave per unit, else the grid cap
     int *slab_shift = nullptr;         // COLUMN_PAIR: smallest and largest shift of each slab, [n0][2] (owned)
-    bkma_optim_params optim{};         // PLAIN: the reference's dispatch for its AdaptiveWg executor
+    bkma_optim_params optim{};         // PLAIN, HYBRID: the reference's dispatch for its AdaptiveWg executor
+    // HYBRID: slabs [0, k_local) with their lines' scratch in LDS (optim),
+    // [k_local, n0) with a global scratch (optim_global, gscratch), the two
+    // launches concurrent on the caller's stream and a side stream
+    u64 k_local = 0;
+    bkma_optim_params optim_global{};
+    bkma_span3d gscratch{nullptr, 0, 0, 0};  // owned
+    hipStream_t side = nullptr;              // owned
+    hipEvent_t fork = nullptr, join = nullptr;  // owned
     int pair_ahead = 0;                // COLUMN_PAIR: largest stage lag, 3 - the smallest shift
     // LINE_LDS
     LdsArgs lds{};
@@ -122,6 +130,10 @@ struct TunedPlan {
     ~TunedPlan() {
         if (halo) (void)hipFree(halo);
         if (slab_shift) (void)hipFree(slab_shift);
+        if (gscratch.data) (void)hipFree(gscratch.data);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
     }
 };
 
@@ -324,6 +336,22 @@ bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s, int ns) 
         const bkma_span3d none{nullptr, 0, 0, 0};
         return bkma_adv_run(BKMA_IMPL_ADAPTIVE_WG, pl.dtype, data, &pl.params, &pl.optim, none, s);
     }
+    if (pl.cfg.strategy == BKMA_STRATEGY_HYBRID) {
+        const bkma_span3d none{nullptr, 0, 0, 0};
+        const u64 n0 = pl.params.n0, kg = n0 - pl.k_local;
+        double *d = (double *)data;
+        if (pl.k_local == 0) return adaptivewg_slabs(d, pl.params, 0, n0, pl.optim_global, pl.gscratch, s);
+        if (kg == 0) return adaptivewg_slabs(d, pl.params, 0, n0, pl.optim, none, s);
+        // fork: the global-scratch slabs run on the side stream beside the
+        // LDS slabs; join before anything later on the caller's stream
+        BKMA_HIP_TRY(hipEventRecord(pl.fork, s));
+        BKMA_HIP_TRY(hipStreamWaitEvent(pl.side, pl.fork, 0));
+        bkma_status st = adaptivewg_slabs(d, pl.params, pl.k_local, kg, pl.optim_global, pl.gscratch, pl.side);
+        if (st == BKMA_OK) st = adaptivewg_slabs(d, pl.params, 0, pl.k_local, pl.optim, none, s);
+        BKMA_HIP_TRY(hipEventRecord(pl.join, pl.side));
+        BKMA_HIP_TRY(hipStreamWaitEvent(s, pl.join, 0));
+        return st;
+    }
     if (pl.cfg.strategy == BKMA_STRATEGY_LINE_LDS)
         return pl.dtype == BKMA_F64 ? launch_lds<double>(pl, data, s, ns) : launch_lds<float>(pl, data, s, ns);
     if (ns < 1 || ns > pl.variant->nsmax) return fail(BKMA_ERR_RUNTIME, "tuned advection: steps per pass out of range");
@@ -425,6 +453,49 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
         if (pl->optim.mem_space != BKMA_MEM_LOCAL)
             return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: PLAIN needs the line in LDS");
         pl->name = "plain_adaptivewg<pref" + std::to_string(cfg.seg_rows) + ">";
+        return BKMA_OK;
+    }
+    if (cfg.strategy == BKMA_STRATEGY_HYBRID) {
+        // HybridMem (src/core/experiments/HybridMem.cpp:4-32, its constructor
+        // :56-98): a fraction percent_loc of the slabs keeps each line's
+        // scratch in LDS, the rest use a global scratch with work-groups sized
+        // without the LDS limit; k_local = floor(n0 * percent_loc)
+        // (dispatch_kernels, src/core/bkma_tools.hpp:157-165)
+        if (dt != BKMA_F64) return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: HYBRID runs FP64 storage only");
+        if (steps_per_pass(cfg) != 1) return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: HYBRID runs one step per pass");
+        if (cfg.seg_rows <= 0 || cfg.chunk_segs < 0 || cfg.chunk_segs > 100)
+            return fail(BKMA_ERR_INVALID_ARGUMENT,
+                        "tuned advection: HYBRID needs seg_rows = pref_wg_size > 0 and chunk_segs = percent in [0, 100]");
+        pl->cfg.steps_per_pass = 1;
+        pl->k_local = p.n0 * (u64)cfg.chunk_segs / 100;
+        const u64 kg = p.n0 - pl->k_local;
+        if (pl->k_local > 0) {
+            BKMA_TRY(bkma_create_optim_params(pl->k_local, p.n1, p.n2, (u64)cfg.seg_rows, 1, 1, 8, 0, &pl->optim));
+            if (pl->optim.mem_space != BKMA_MEM_LOCAL)
+                return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: HYBRID needs the LDS slabs' lines in LDS");
+        }
+        if (kg > 0) {
+            // set_ideal_sizes + adjust_sizes_mem_limit(INT_MAX, n1): no LDS limit
+            BKMA_TRY(bkma_create_optim_params(kg, p.n1, p.n2, (u64)cfg.seg_rows, 1, 1, 8, (u64)INT32_MAX * 8,
+                                              &pl->optim_global));
+            pl->optim_global.mem_space = BKMA_MEM_GLOBAL;
+            uint64_t e0, e1, e2;
+            BKMA_TRY(bkma_scratch_extents(BKMA_IMPL_ADAPTIVE_WG, &pl->optim_global, kg, p.n1, p.n2, 1, &e0, &e1, &e2));
+            if (hipMalloc(&pl->gscratch.data, std::max<u64>(e0 * e1 * e2 * 8, 16)) != hipSuccess) {
+                pl->gscratch.data = nullptr;
+                return fail(BKMA_ERR_OUT_OF_MEMORY, "tuned advection: HYBRID global scratch allocation failed");
+            }
+            pl->gscratch.e0 = e0;
+            pl->gscratch.e1 = e1;
+            pl->gscratch.e2 = e2;
+        }
+        if (pl->k_local > 0 && kg > 0) {
+            BKMA_HIP_TRY(hipStreamCreateWithFlags(&pl->side, hipStreamNonBlocking));
+            BKMA_HIP_TRY(hipEventCreateWithFlags(&pl->fork, hipEventDisableTiming));
+            BKMA_HIP_TRY(hipEventCreateWithFlags(&pl->join, hipEventDisableTiming));
+        }
+        pl->name = "hybrid_adaptivewg<pref" + std::to_string(cfg.seg_rows) + ",local" +
+                   std::to_string(cfg.chunk_segs) + "%>";
         return BKMA_OK;
     }
     if (cfg.strategy == BKMA_STRATEGY_LINE_LDS) {
@@ -541,6 +612,21 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
             c.seg_rows = pref;
             out.push_back(c);
         }
+        // local/global splits of the slabs (HybridMem), at pref_wg_size 256
+        for (int pct : {25, 50, 75}) {
+            const u64 kl = p.n0 * (u64)pct / 100;
+            bkma_optim_params o;
+            if (kl == 0 || kl == p.n0 || bkma_create_optim_params(kl, p.n1, p.n2, 256, 1, 1, 8, 0, &o) != BKMA_OK ||
+                o.mem_space != BKMA_MEM_LOCAL)
+                continue;
+            bkma_tuned_config c;
+            std::memset(&c, 0, sizeof(c));
+            c.strategy = BKMA_STRATEGY_HYBRID;
+            c.steps_per_pass = 1;
+            c.seg_rows = 256;
+            c.chunk_segs = pct;
+            out.push_back(c);
+        }
     }
     return out;
 }
@@ -554,6 +640,7 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
 int heuristic_score(const bkma_tuned_config &c, bkma_dtype dt, const bkma_adv_params &p) {
     if (c.strategy == BKMA_STRATEGY_PLAIN)  // short lines: one work-group per slab would idle
         return (p.n1 < 64 ? 900000000 : 10000000) - std::abs(c.seg_rows - 256);
+    if (c.strategy == BKMA_STRATEGY_HYBRID) return 5000000 + c.chunk_segs;
     if (c.strategy == BKMA_STRATEGY_COLUMN_PAIR) {
         // FP32: DPP broadcast, FP64 stage-0 window, VEC 2, P 14, 5 steps;
         // FP64: VEC 2, P 14, 4 steps

@@ -224,4 +224,11 @@ template <class T> struct ConvSolverDev {
     }
 };
 
+// The reference's AdaptiveWg executor (FP64 storage) over slabs
+// [i0_begin, i0_begin + count) of the [n0][n1][n2] array at data: `o` is the
+// dispatch created for `count` slabs, its mem_space chooses LDS or the
+// caller's global scratch.  Used by the tuned plans' local/global splits.
+bkma_status adaptivewg_slabs(double *data, const bkma_adv_params &p, u64 i0_begin, u64 count,
+                             const bkma_optim_params &o, const bkma_span3d &scratch, hipStream_t stream);
+
 }  // namespace bkma

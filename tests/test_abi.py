@@ -42,6 +42,14 @@ def test_binding_covers_header():
     assert _declared_functions() <= set(B.EXPORTS)
 
 
+def test_binding_strategy_codes_match_header():
+    text = open(os.path.join(ROOT, "include", "bkma_amd.h")).read()
+    codes = {m.group(1): int(m.group(2)) for m in re.finditer(r"\bBKMA_(STRATEGY_\w+)\s*=\s*(\d+)", text)}
+    assert len(codes) == 5
+    for name, value in codes.items():
+        assert getattr(B, name) == value, name
+
+
 def test_update_deltas():
     p = B.make_params(256, 1024, 1, dt=1e-3, minRealVx=-1, maxRealVx=1)
     assert p.dx == 2.0 ** -10 and p.inv_dx == 1024.0 and p.realWidthX == 1.0 and p.dvx == 2 / 256

@@ -195,6 +195,53 @@ def test_plain_strategy_is_fp64_only(cuda):
         B.AdvPlan(p, B.F32, config=_plain_config(256))
 
 
+def _hybrid_config(pref, percent):
+    c = B.TunedConfig()
+    c.strategy = B.STRATEGY_HYBRID
+    c.seg_rows = pref
+    c.chunk_segs = percent
+    c.steps_per_pass = 1
+    return c
+
+
+HYBRID_CASES = [
+    # (n0, n1, n2, pref_wg_size, percent of slabs with the scratch in LDS)
+    (7, 300, 24, 256, 50),     # 3 LDS slabs, 4 global-scratch slabs, concurrent
+    (7, 300, 24, 128, 30),     # 2 and 5
+    (9, 200, 130, 256, 75),    # n2 above pref_wg_size: w2 = pref
+    (5, 300, 24, 256, 100),    # every slab in LDS
+    (5, 300, 24, 256, 0),      # every slab with the global scratch
+    (3, 24000, 2, 256, 0),     # lines beyond the LDS: global scratch only
+]
+
+
+@pytest.mark.parametrize("case", HYBRID_CASES, ids=lambda c: "x".join(map(str, c[:3])) + f"-w{c[3]}-loc{c[4]}")
+def test_hybrid_strategy_bit_exact(cuda, case):
+    """HYBRID splits the slabs between the AdaptiveWg executor with LDS
+    scratch and with a global scratch: bit-identical to the oracle."""
+    n0, n1, n2, pref, pct = case
+    p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    po = O.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    host = O.uniform(n0 * n1 * n2, 10).reshape(n0, n1, n2)
+    got, spp = _run(host, p, B.F64, 4, config=_hybrid_config(pref, pct))
+    assert spp == 1
+    ref = host.copy()
+    O.advect_f64(ref, po, 4)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+def test_hybrid_rejections(cuda):
+    p = B.make_params(5, 300, 24, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    with pytest.raises(B.BkmaUnsupported):
+        B.AdvPlan(p, B.F32, config=_hybrid_config(256, 50))
+    with pytest.raises(B.BkmaError):
+        B.AdvPlan(p, B.F64, config=_hybrid_config(256, 101))
+    # lines beyond the LDS cannot keep their scratch there
+    q = B.make_params(4, 24000, 2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    with pytest.raises(B.BkmaUnsupported):
+        B.AdvPlan(q, B.F64, config=_hybrid_config(256, 50))
+
+
 @pytest.mark.parametrize("dtype", [B.F64, B.F32])
 def test_short_lines(cuda, dtype):
     """Lines of 16 rows (the reference benchmark's n1 = 16 shapes): the
