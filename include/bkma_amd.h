@@ -200,7 +200,7 @@ typedef struct bkma_tuned_config {
     int32_t vec;          /* consecutive i2 cells per lane */
     int32_t smax;         /* largest |v*dt/dx| the stencil window covers */
     int32_t prefetch;     /* rows loaded ahead of the stencil window */
-    int32_t waves_per_wg; /* wavefronts per work-group */
+    int32_t waves_per_wg; /* wavefronts per work-group (LINE_LDS: 4, 8 or 16; 0 = 4) */
     int32_t nontemporal;  /* 1: non-temporal global loads/stores */
     int32_t persistent;   /* 1: grid capped at the resident work-groups, waves loop over units */
     int32_t seg_rows;     /* LINE_LDS: rows per segment (0 = whole line in LDS); PLAIN, HYBRID: pref_wg_size */

@@ -281,9 +281,9 @@ __device__ __forceinline__ void line_steps_unit(T *__restrict__ slab, const AdvD
     step((const double *)(ns % 2 == 0 ? w0 : w1), slab);
 }
 
-template <class T, bool SEG>
-__global__ __launch_bounds__(256) void adv_line_lds_kernel(T *__restrict__ data, const T *__restrict__ halo,
-                                                           LdsArgs a) {
+template <class T, bool SEG, int NT>
+__global__ __launch_bounds__(NT) void adv_line_lds_kernel(T *__restrict__ data, const T *__restrict__ halo,
+                                                          LdsArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (!SEG && (a.ns > 1 || a.slabs > 1)) {
         // whole lines of `slabs` slabs, ns steps: [slabs][n1][n2] T, then two
@@ -477,9 +477,10 @@ __device__ __forceinline__ void line_chunk_steps_unit(T *__restrict__ slab, cons
     }
 }
 
-template <class T>
-__global__ __launch_bounds__(256) void adv_line_chunk_kernel(T *__restrict__ data, const T *__restrict__ halo,
-                                                             LdsArgs a) {
+// NT threads per work-group; FUSED: ns > 1 steps per pass (a.ns), else one.
+template <class T, int NT, bool FUSED>
+__global__ __launch_bounds__(NT) void adv_line_chunk_kernel(T *__restrict__ data, const T *__restrict__ halo,
+                                                            LdsArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     // fused steps: the staged rows (storage type), then two FP64 buffers of as many rows
     const i64 rows = a.ns * (a.hb_max + a.ha_max) + a.seg_rows;
@@ -491,7 +492,7 @@ __global__ __launch_bounds__(256) void adv_line_chunk_kernel(T *__restrict__ dat
         const i64 c0 = ch * chunk_rows;
         const i64 c1 = min(a.p.n1, c0 + chunk_rows);
         const T *hs = halo + unit * a.ns * (a.hb_max + a.ha_max) * a.p.n2;
-        if (a.ns > 1)
+        if constexpr (FUSED)
             line_chunk_steps_unit<T>(data + i0 * a.p.n1 * a.p.n2, hs, a.p, i0, c0, c1, a.seg_rows, a.hb_max,
                                      a.ha_max, (int)a.ns, st, w0, w0 + rows * a.p.n2);
         else

@@ -126,6 +126,7 @@ struct TunedPlan {
     // LINE_LDS
     LdsArgs lds{};
     u64 lds_bytes = 0;
+    int lds_threads = 256;  // threads per work-group: 256, 512 or 1024
     void *halo = nullptr;  // segmented mode: halo snapshot buffer (owned)
     ~TunedPlan() {
         if (halo) (void)hipFree(halo);
@@ -305,8 +306,8 @@ bkma_status lds_shape(const bkma_adv_params &p, bkma_dtype dt, i64 seg_rows, i64
     return BKMA_OK;
 }
 
-template <class T>
-bkma_status launch_lds(const TunedPlan &pl, void *data, hipStream_t s, int ns) {
+template <class T, int NT>
+bkma_status launch_lds_nt(const TunedPlan &pl, void *data, hipStream_t s, int ns) {
     LdsArgs a = pl.lds;
     a.ns = ns;
     const i64 nunits = a.slabs > 1 ? ((i64)pl.params.n0 + a.slabs - 1) / a.slabs : (i64)pl.params.n0 * a.nseg;
@@ -319,15 +320,26 @@ bkma_status launch_lds(const TunedPlan &pl, void *data, hipStream_t s, int ns) {
         const unsigned cunits = (unsigned)std::min<i64>((i64)pl.params.n0 * a.nchunk, kLdsMaxGrid);
         hipLaunchKernelGGL(halo_snapshot_kernel<T>, dim3(cunits), dim3(256), 0, s, (const T *)data, (T *)pl.halo, sa);
         BKMA_TRY(check_launch("halo_snapshot_kernel"));
-        auto k = adv_line_chunk_kernel<T>;
+        auto k = ns > 1 ? adv_line_chunk_kernel<T, NT, true> : adv_line_chunk_kernel<T, NT, false>;
         BKMA_HIP_TRY(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds_bytes));
-        hipLaunchKernelGGL(k, dim3(cunits), dim3(256), (size_t)pl.lds_bytes, s, (T *)data, (const T *)pl.halo, a);
+        hipLaunchKernelGGL(k, dim3(cunits), dim3(NT), (size_t)pl.lds_bytes, s, (T *)data, (const T *)pl.halo, a);
     } else {
-        auto k = adv_line_lds_kernel<T, false>;
+        auto k = adv_line_lds_kernel<T, false, NT>;
         BKMA_HIP_TRY(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds_bytes));
-        hipLaunchKernelGGL(k, dim3(units), dim3(256), (size_t)pl.lds_bytes, s, (T *)data, (const T *)nullptr, a);
+        hipLaunchKernelGGL(k, dim3(units), dim3(NT), (size_t)pl.lds_bytes, s, (T *)data, (const T *)nullptr, a);
     }
     return check_launch("adv_line_lds_kernel");
+}
+
+// LINE_LDS work-groups of 4, 8 or 16 wavefronts (one work-group per CU when
+// the staged rows take most of the LDS: more waves hide the HBM latency)
+template <class T>
+bkma_status launch_lds(const TunedPlan &pl, void *data, hipStream_t s, int ns) {
+    switch (pl.lds_threads) {
+        case 512: return launch_lds_nt<T, 512>(pl, data, s, ns);
+        case 1024: return launch_lds_nt<T, 1024>(pl, data, s, ns);
+        default: return launch_lds_nt<T, 256>(pl, data, s, ns);
+    }
 }
 
 // One pass of ns steps (ns <= the plan's steps per pass).
@@ -386,13 +398,13 @@ std::string stream_name(const StreamVariant &v, int ns, bool persistent) {
            (v.dpp ? ",dpp" : "") + (persistent ? ",persistent" : "") + ">";
 }
 
-std::string lds_name(bkma_dtype dt, const LdsArgs &a) {
+std::string lds_name(bkma_dtype dt, const LdsArgs &a, int threads) {
     return std::string("line_lds<") + (dt == BKMA_F64 ? "f64" : "f32") +
            (a.nseg > 1 ? ",seg" + std::to_string(a.seg_rows) + "x" + std::to_string(a.nseg) + ",chunk" +
                              std::to_string(a.chunk_segs)
                        : ",whole_line") +
            (a.slabs > 1 ? ",slabs" + std::to_string(a.slabs) : "") + (a.ns > 1 ? ",steps" + std::to_string(a.ns) : "") +
-           ">";
+           (threads != 256 ? ",wg" + std::to_string(threads) : "") + ">";
 }
 
 // Resident work-groups on the whole device for a variant (persistent grid).
@@ -504,6 +516,11 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
             return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: LINE_LDS fuses at most 4 steps per pass");
         pl->cfg.steps_per_pass = ns;
         pl->ns = ns;
+        const int wpb = cfg.waves_per_wg > 0 ? cfg.waves_per_wg : 4;
+        if (wpb != 4 && wpb != 8 && wpb != 16)
+            return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: LINE_LDS work-groups have 4, 8 or 16 wavefronts");
+        pl->lds_threads = 64 * wpb;
+        pl->cfg.waves_per_wg = wpb;
         if (lds_shape(p, dt, cfg.seg_rows, cfg.chunk_segs, ns, &pl->lds, &pl->lds_bytes, &why) != BKMA_OK)
             return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
         if (pl->lds.nseg > 1) {
@@ -517,7 +534,7 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
         }
         pl->cfg.seg_rows = pl->lds.nseg > 1 ? (int32_t)pl->lds.seg_rows : 0;
         pl->cfg.chunk_segs = (int32_t)(pl->lds.nseg > 1 ? pl->lds.chunk_segs : pl->lds.slabs);
-        pl->name = lds_name(dt, pl->lds);
+        pl->name = lds_name(dt, pl->lds, pl->lds_threads);
         return BKMA_OK;
     }
     return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: unknown strategy");
@@ -565,7 +582,8 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
     }
     const u64 esz = dt == BKMA_F64 ? 8 : 4;
     LdsArgs a;
-    u64 bytes;
+    u64 bytes, lds_max = 0;
+    if (lds_limit_bytes(&lds_max) != BKMA_OK) lds_max = 0;
     int dlo = 0, dhi = 0;
     if (params_finite(p)) shift_range(p, &dlo, &dhi);
     const i64 halo_rows = std::max(0, dhi + 2) + std::max(0, 3 - dlo);
@@ -589,11 +607,17 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
                 c.steps_per_pass = ns;
                 c.seg_rows = (int32_t)(a.nseg > 1 ? a.seg_rows : 0);
                 c.chunk_segs = (int32_t)(a.nseg > 1 ? a.chunk_segs : a.slabs);
-                bool dup = false;
-                for (auto &o : out)
-                    dup |= o.strategy == c.strategy && o.seg_rows == c.seg_rows && o.chunk_segs == c.chunk_segs &&
-                           o.steps_per_pass == c.steps_per_pass;
-                if (!dup) out.push_back(c);
+                // a work-group that takes over a quarter of the LDS leaves
+                // few waves per CU at 4 per work-group: 8 and 16 too
+                for (int wpb : {4, 8, 16}) {
+                    if (wpb > 4 && bytes * 4 <= lds_max) break;
+                    c.waves_per_wg = wpb;
+                    bool dup = false;
+                    for (auto &o : out)
+                        dup |= o.strategy == c.strategy && o.seg_rows == c.seg_rows && o.chunk_segs == c.chunk_segs &&
+                               o.steps_per_pass == c.steps_per_pass && o.waves_per_wg == c.waves_per_wg;
+                    if (!dup) out.push_back(c);
+                }
             }
         }
     }
@@ -656,8 +680,11 @@ int heuristic_score(const bkma_tuned_config &c, bkma_dtype dt, const bkma_adv_pa
         return 1000000000 - c.smax * 1000000 - std::abs(ns - want) * 100000 + c.vec * 1000 + c.prefetch * 10 +
                c.nontemporal - c.persistent - 100 * c.wide_window;
     }
-    return c.seg_rows == 0 ? 500000000 - std::abs(steps_per_pass(c) - 2) * 1000
-                           : 100000000 + std::min(c.seg_rows, 1 << 24) / 1024 - std::abs(c.chunk_segs - 16) * 10;
+    // LINE_LDS: whole lines; else the longest segment, with 16-wave
+    // work-groups where they are offered (the segment fills the LDS)
+    return c.seg_rows == 0 ? 500000000 - std::abs(steps_per_pass(c) - 2) * 1000 - c.waves_per_wg
+                           : 100000000 + std::min(c.seg_rows, 1 << 24) / 1024 - std::abs(c.chunk_segs - 16) * 10 +
+                                 (c.waves_per_wg == 16 ? 5 : 0);
 }
 
 bkma_status time_plan(const TunedPlan &pl, void *buf, int reps, hipStream_t s, float *ms) {

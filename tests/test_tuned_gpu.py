@@ -437,6 +437,48 @@ def test_line_lds_segments_fused_steps(cuda, dtype, case, ns):
     assert np.array_equal(got.view(bits), ref.view(bits)), np.abs(got.astype(np.float64) - ref).max()
 
 
+WG_CASES = [
+    # (n0, n1, n2, dt, seg_rows, chunk_segs, steps per pass)
+    (3, 4096, 8, 2e-3, 500, 3, 1),   # chained segments
+    (3, 4096, 8, 2e-3, 500, 3, 3),   # chained segments, fused steps
+    (2, 16384, 4, 1e-3, 1024, 4, 1),
+    (3, 200, 8, 2e-3, 0, 1, 1),      # whole lines
+    (300, 16, 1, 2e-3, 0, 7, 4),     # whole lines, slab groups, fused steps
+]
+
+
+@pytest.mark.parametrize("wpb", [8, 16])
+@pytest.mark.parametrize("case", WG_CASES, ids=lambda c: "x".join(map(str, c[:3])) + f"-seg{c[4]}-chunk{c[5]}-ns{c[6]}")
+@pytest.mark.parametrize("dtype", [B.F64, B.F32], ids=["f64", "f32"])
+def test_line_lds_work_group_sizes(cuda, dtype, case, wpb):
+    """LINE_LDS with 512- and 1024-thread work-groups: same results as the
+    oracle (7 steps: full passes and a shorter last one)."""
+    n0, n1, n2, dt, seg, chunk, ns = case
+    p = B.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
+    po = O.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
+    np_dt = np.float64 if dtype == B.F64 else np.float32
+    host = O.uniform(n0 * n1 * n2, 12, np_dt).reshape(n0, n1, n2)
+    c = _lds_seg_config(ns, seg, chunk)
+    c.waves_per_wg = wpb
+    got, spp = _run(host, p, dtype, 7, config=c)
+    assert spp == ns
+    ref = host.copy()
+    if dtype == B.F64:
+        O.advect_f64(ref, po, 7)
+    else:
+        O.advect_f32_blocked(ref, po, 7, ns)
+    bits = np.uint64 if dtype == B.F64 else np.uint32
+    assert np.array_equal(got.view(bits), ref.view(bits))
+
+
+def test_line_lds_work_group_size_checked(cuda):
+    p = B.make_params(3, 200, 8, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    c = _lds_config(1)
+    c.waves_per_wg = 6
+    with pytest.raises(B.BkmaUnsupported):
+        B.AdvPlan(p, B.F32, config=c)
+
+
 def test_line_lds_chained_fused_steps_need_segments_past_the_halo(cuda):
     # v in {-1, -0.5, 0, 0.5}: backward shifts up to ~21 cells need 23 halo
     # rows a step; 4 steps reach 92 rows, beyond 64-row segments, 2 steps fit
