@@ -41,8 +41,8 @@ struct BatchItem {
     i64 i0;
 };
 
-template <class T>
-__global__ __launch_bounds__(256) void adv_batch_kernel(const BatchProblemDev *__restrict__ probs,
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void adv_batch_kernel(const BatchProblemDev *__restrict__ probs,
                                                         const BatchItem *__restrict__ items, i64 nitems,
                                                         unsigned long long *counter) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -122,6 +122,7 @@ struct AdvBatch {
     unsigned long long *d_counter = nullptr;
     std::vector<void *> halos;
     u64 lds_budget = 0;
+    int threads = 256;  // per work-group: 256 or 1024 (the autotuner's choice)
     unsigned grid = 0;
     hipStream_t stream = nullptr;  // internal stream (graph capture and replay)
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -149,9 +150,9 @@ template <class T> static bkma_status batch_step(const AdvBatch &b, hipStream_t 
                            (const BatchProblemDev *)b.d_probs, (const BatchItem *)b.d_seg_items);
         BKMA_TRY(check_launch("batch_halo_kernel"));
     }
-    hipLaunchKernelGGL(adv_batch_kernel<T>, dim3(b.grid), dim3(256), (size_t)b.lds_budget, s,
-                       (const BatchProblemDev *)b.d_probs, (const BatchItem *)b.d_items, (i64)b.host_items.size(),
-                       b.d_counter);
+    auto k = b.threads == 1024 ? adv_batch_kernel<T, 1024> : adv_batch_kernel<T, 256>;
+    hipLaunchKernelGGL(k, dim3(b.grid), dim3(b.threads), (size_t)b.lds_budget, s, (const BatchProblemDev *)b.d_probs,
+                       (const BatchItem *)b.d_items, (i64)b.host_items.size(), b.d_counter);
     return check_launch("adv_batch_kernel");
 }
 
@@ -161,8 +162,9 @@ static bkma_status batch_steps(const AdvBatch &b, u64 nsteps, hipStream_t s) {
     return BKMA_OK;
 }
 
-// Build the item lists for an LDS budget (bytes per work-group).
-static bkma_status build_items(AdvBatch &b, u64 budget) {
+// Build the item lists for an LDS budget (bytes per work-group) and
+// work-groups of `threads` threads.
+static bkma_status build_items(AdvBatch &b, u64 budget, int threads) {
     const u64 esz = b.dtype == BKMA_F64 ? 8 : 4;
     for (void *h : b.halos) (void)hipFree(h);
     b.halos.clear();
@@ -234,12 +236,17 @@ static bkma_status build_items(AdvBatch &b, u64 budget) {
     }
     // Persistent grid: every work-group that can be resident at this LDS size.
     b.lds_budget = budget;
+    b.threads = threads;
     int dev = 0, cus = 0, blocks = 0;
     BKMA_HIP_TRY(hipGetDevice(&dev));
     BKMA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const void *k = b.dtype == BKMA_F64 ? (const void *)adv_batch_kernel<double> : (const void *)adv_batch_kernel<float>;
+    const void *k = b.dtype == BKMA_F64
+                        ? (threads == 1024 ? (const void *)adv_batch_kernel<double, 1024>
+                                           : (const void *)adv_batch_kernel<double, 256>)
+                        : (threads == 1024 ? (const void *)adv_batch_kernel<float, 1024>
+                                           : (const void *)adv_batch_kernel<float, 256>);
     BKMA_HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)budget));
-    BKMA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 256, (size_t)budget));
+    BKMA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, threads, (size_t)budget));
     b.grid = (unsigned)std::max<i64>(1, std::min<i64>((i64)blocks * cus, (i64)b.host_items.size()));
     // Fresh counter (the modulus changed with the item count / grid).
     BKMA_HIP_TRY(hipMemset(b.d_counter, 0, sizeof(unsigned long long)));
@@ -295,36 +302,42 @@ extern "C" bkma_status bkma_adv_batch_create(const bkma_adv_problem *problems, i
     std::vector<u64> budgets = {std::min<u64>(65536, cap)};
     if (autotune) budgets = {32768, 65536, 98304, cap};
     u64 best_budget = 0;
+    int best_threads = 256;
     float best_ms = 1e30f;
+    bool done = false;
     for (u64 bud : budgets) {
-        if (bud > lds_max) continue;
-        if (build_items(*b, bud) != BKMA_OK) continue;
-        if (!autotune) {
-            best_budget = bud;
-            break;
+        for (int thr : {256, 1024}) {
+            if (done || bud > lds_max) break;
+            if (build_items(*b, bud, thr) != BKMA_OK) continue;
+            if (!autotune) {
+                best_budget = bud;
+                done = true;
+                break;
+            }
+            // The timing runs update the problems in place: the autotuner is
+            // meant to run before the data is initialised (or on scratch data).
+            hipEvent_t e0, e1;
+            (void)hipEventCreate(&e0);
+            (void)hipEventCreate(&e1);
+            st = batch_steps(*b, 1, b->stream);
+            (void)hipEventRecord(e0, b->stream);
+            if (st == BKMA_OK) st = batch_steps(*b, 3, b->stream);
+            (void)hipEventRecord(e1, b->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            if (st == BKMA_OK && ms < best_ms) {
+                best_ms = ms;
+                best_budget = bud;
+                best_threads = thr;
+            }
+            st = BKMA_OK;
         }
-        // The timing runs update the problems in place: the autotuner is
-        // meant to run before the data is initialised (or on scratch data).
-        hipEvent_t e0, e1;
-        (void)hipEventCreate(&e0);
-        (void)hipEventCreate(&e1);
-        st = batch_steps(*b, 1, b->stream);
-        (void)hipEventRecord(e0, b->stream);
-        if (st == BKMA_OK) st = batch_steps(*b, 3, b->stream);
-        (void)hipEventRecord(e1, b->stream);
-        (void)hipEventSynchronize(e1);
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (st == BKMA_OK && ms < best_ms) {
-            best_ms = ms;
-            best_budget = bud;
-        }
-        st = BKMA_OK;
     }
     if (best_budget == 0) return bail(fail(BKMA_ERR_UNSUPPORTED, "advection batch: no LDS budget covers the problems"));
-    st = build_items(*b, best_budget);
+    st = build_items(*b, best_budget, best_threads);
     if (st != BKMA_OK) return bail(st);
     (void)stream;
     *out = b;
