@@ -49,7 +49,10 @@ def parse():
                     help="only run a plain 1 GiB device copy (3 launches), for scaling rocprofv3 byte counters")
     ap.add_argument("--extra", default="",
                     help="comma list of extra measurements added to the JSON line: conv, batch, conv_fp8, long")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.steps < 1 or args.warmup < 0:
+        ap.error("--steps must be at least 1 and --warmup at least 0")
+    return args
 
 
 def cpu_baseline(target_s: float):
@@ -141,8 +144,11 @@ def main():
     torch.cuda.synchronize()
     wall32 = time.perf_counter() - t0
     launch_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(len(passes))]
-    full_ms = [t for t, k in zip(launch_ms, passes) if k == spp]
-    kern_ms32 = sum(full_ms) / len(full_ms)  # average duration of a full pass (the dominant launch)
+    # the roofline object describes the longest launch of the run: a full
+    # pass of spp steps (fewer only when --steps is below spp)
+    spp_l = max(passes)
+    full_ms = [t for t, k in zip(launch_ms, passes) if k == spp_l]
+    kern_ms32 = sum(full_ms) / len(full_ms)  # average duration of that launch
     gcell32 = cells * args.steps / wall32 / 1e9
 
     # ---- plain FP64 GPU (reference executor and policy) ----------------
@@ -176,14 +182,14 @@ def main():
     del g64
 
     # algorithmic bytes: 8 B (one FP32 read and one write) per cell-update, and
-    # one launch makes spp cell-updates of every cell
-    bytes_alg = cells * spp * 2 * 4
+    # one launch makes spp_l cell-updates of every cell
+    bytes_alg = cells * spp_l * 2 * 4
     achieved = bytes_alg / (kern_ms32 * 1e-3) / 1e9
     # the bytes a launch must move at least (each cell read and written once)
     bytes_hbm = cells * 2 * 4
     hbm_gbs = bytes_hbm / (kern_ms32 * 1e-3) / 1e9
     # the same launch against the vector FP64 peak (the stencil sums run in FP64)
-    fp64_tflops = cells * spp * FLOP_PER_CELL_STEP / (kern_ms32 * 1e-3) / 1e12
+    fp64_tflops = cells * spp_l * FLOP_PER_CELL_STEP / (kern_ms32 * 1e-3) / 1e12
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
 
     out = {
@@ -207,7 +213,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": load_traffic(args.traffic, name32) if n0 == N0 else None,
                      "kernel_avg_ms": round(kern_ms32, 5), "algorithmic_bytes_per_launch": bytes_alg,
-                     "steps_per_launch": spp,
+                     "steps_per_launch": spp_l,
                      "fp64_valu": {"achieved_TFLOPs": round(fp64_tflops, 2), "peak": FP64_VALU_PEAK_TFLOPS,
                                    "frac": round(fp64_tflops / FP64_VALU_PEAK_TFLOPS, 4),
                                    "flop_per_cell_step": FLOP_PER_CELL_STEP},

@@ -611,6 +611,7 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
                 // few waves per CU at 4 per work-group: 8 and 16 too
                 for (int wpb : {4, 8, 16}) {
                     if (wpb > 4 && bytes * 4 <= lds_max) break;
+                    if (wpb == 16 && ns > 1) break;  // the fused-step kernel spills at 1024 threads
                     c.waves_per_wg = wpb;
                     bool dup = false;
                     for (auto &o : out)
