@@ -94,7 +94,7 @@ def main():
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the advection path is "
               f"embarrassingly parallel over i0 slabs and would shard n0 across GPUs with no per-step "
-              f"communication, but this build runs and reports a single MI355X.", file=sys.stderr)
+              f"communication, but bench.py runs and reports a single MI355X.", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -35,7 +35,7 @@ typedef enum bkma_status {
     BKMA_ERR_INVALID_ARGUMENT = 1, /* reference: std::invalid_argument */
     BKMA_ERR_RUNTIME = 2,          /* reference: std::runtime_error */
     BKMA_ERR_DEVICE = 3,           /* HIP runtime failure (reference: sycl::exception) */
-    BKMA_ERR_UNSUPPORTED = 4,      /* shape/dtype combination this build does not run */
+    BKMA_ERR_UNSUPPORTED = 4,      /* shape/dtype combination this library does not run */
     BKMA_ERR_OUT_OF_MEMORY = 5
 } bkma_status;
 

@@ -16,9 +16,9 @@ namespace bkma {
 inline int pick_device(bool run_on_gpu) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
-        throw device_error("no HIP GPU available; this build has no CPU device");
+        throw device_error("no HIP GPU available; this library has no CPU device");
     if (!run_on_gpu)
-        std::cout << "CPU execution was requested; this build runs the kernels on GPU 0\n" << std::endl;
+        std::cout << "CPU execution was requested; the kernels run on GPU 0\n" << std::endl;
     return 0;
 }
 

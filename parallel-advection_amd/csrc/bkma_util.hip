@@ -313,7 +313,7 @@ extern "C" bkma_status bkma_create_optim_params(uint64_t n0, uint64_t n1, uint64
         w0 = 1;
         if (w2 == 0) {
             // The reference divides by zero here (line longer than the LDS);
-            // this build keeps one line per work-group and a global scratch.
+            // this library keeps one line per work-group and a global scratch.
             w2 = 1;
             mem_space = BKMA_MEM_GLOBAL;
         }

@@ -4,7 +4,7 @@ test; each one comes from the reference's own tests, a fixture, a closed form
 or an invariant that a plausible transcription mistake breaks.
 
 Pinning map (oracle function -> checks):
-  orc_lag_basis        survey fixture at p=1/3 (exact rationals), nodal values,
+  orc_lag_basis        exact-rational fixture at p=1/3, nodal values,
                        reproduction of degree <= 4 polynomials, reflection
                        symmetry, NOT textbook Lagrange (degree-5 defect)
   orc_adv_foot/_cell   integer-shift known answer (bit-exact circular shift),
@@ -40,8 +40,8 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 # ---------------------------------------------------------------------------
 # lag_basis
 # ---------------------------------------------------------------------------
-def test_lag_basis_survey_fixture():
-    fx = json.load(open(os.path.join(GOLDEN, "lag_basis_survey.json")))
+def test_lag_basis_exact_rational_fixture():
+    fx = json.load(open(os.path.join(GOLDEN, "lag_basis_p1_3_exact.json")))
     p = float(Fraction(fx["p"]))
     c = O.lag_basis(2.0 + p)
     np.testing.assert_allclose(c * 729.0, fx["coef_times_729"], rtol=0, atol=1e-12)
