@@ -531,6 +531,31 @@ def test_zero_steps_is_identity(cuda, dtype):
     assert np.array_equal(got, host)
 
 
+@pytest.mark.parametrize("axis", ["n0", "n1", "n2"])
+def test_empty_extents_are_rejected(cuda, axis):
+    """An empty array is an argument error at every advection entry point
+    (plans with and without the autotuner, the plain executors), never a
+    launch."""
+    torch = _torch()
+    p = B.make_params(4, 64, 8, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    setattr(p, axis, 0)
+    d = torch.zeros(16, dtype=torch.float64, device="cuda")
+    for autotune in (False, True):
+        with pytest.raises(B.BkmaInvalidArgument):
+            B.AdvPlan(p, B.F64, autotune=autotune)
+    with pytest.raises(B.BkmaInvalidArgument):
+        B.AdvPlan(p, B.F32, config=_lds_config(1))
+    o = B.create_optim_params(4, 64, 8, pref_wg_size=128, elem_size=8)
+    for impl in (B.ADAPTIVE_WG, B.ND_RANGE):
+        with pytest.raises(B.BkmaInvalidArgument):
+            B.adv_run(impl, d, p, o)
+    # a batch holding one empty problem beside a valid one
+    q = B.make_params(2, 64, 8, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    e = torch.zeros(2 * 64 * 8, dtype=torch.float64, device="cuda")
+    with pytest.raises(B.BkmaInvalidArgument):
+        B.AdvBatch([e, d], [q, p], B.F64)
+
+
 @pytest.mark.parametrize("shape", [(1, 32, 1), (1, 64, 64), (2, 40, 7), (1, 1024, 4096)])
 @pytest.mark.parametrize("dtype", [B.F64, B.F32], ids=["f64", "f32"])
 def test_small_and_single_slab_shapes(cuda, shape, dtype):
