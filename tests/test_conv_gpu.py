@@ -263,6 +263,29 @@ def test_conv_mfma_persistent_many_tiles(cuda, mode, io, tol):
     assert np.array_equal(got[:, :, Lout:], x_in[:, :, Lout:])
 
 
+@pytest.mark.parametrize("L", [2048, 2047], ids=["L2048", "L2047"])
+def test_conv_bf16_storage_many_tiles(cuda, L):
+    """BF16 storage with vector (L % 4 == 0) and scalar staging loads: 64 x 16
+    tiles are more than the resident work-groups, so the persistent loop, the
+    double-buffered staging and the halo snapshots all run."""
+    torch = _torch()
+    Bn, C, K = 64, 128, 5
+    x, w, b = _case(Bn, C, L, K, 13)
+    tx = torch.from_numpy(x).to(torch.bfloat16).cuda()
+    x_in = tx.double().cpu().numpy()
+    plan = B.Conv1dPlan(B.CONV_BF16, B.BF16, Bn, C, L, K, torch.from_numpy(w).float().cuda(),
+                        torch.from_numpy(b).float().cuda())
+    plan.run(tx)
+    torch.cuda.synchronize()
+    got = tx.double().cpu().numpy()
+    ref = _ref(x_in, w.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64))
+    Lout = L - K + 1
+    rmax, rl2 = _rel(got, ref, Lout)
+    print(plan.name(), f"rel_max {rmax:.3e} rel_L2 {rl2:.3e}", flush=True)
+    assert rmax <= 1e-2
+    assert np.array_equal(got[:, :, Lout:], x_in[:, :, Lout:])
+
+
 @pytest.mark.parametrize("shape", [(40, 128, 1024, 5), (20, 256, 1024, 7)], ids=lambda s: "x".join(map(str, s)))
 def test_conv_fp8_persistent_many_tiles_exact(cuda, shape):
     torch = _torch()
