@@ -12,6 +12,8 @@ const StreamVariant *pair_variants_f64(int *count) {
         BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 3, 14, 1, true, false, 4),
 #else
         BKMA_PAIR_VARIANT(double, double, BKMA_F64, 2, 3, 14, 1, true, false, 5),
+        // one column per lane: the window of more fused steps fits (195 VGPRs at 8)
+        BKMA_PAIR_VARIANT(double, double, BKMA_F64, 1, 3, 14, 1, true, false, 8),
 #endif
     };
     *count = (int)(sizeof(table) / sizeof(table[0]));
