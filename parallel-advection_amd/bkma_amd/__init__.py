@@ -134,8 +134,6 @@ EXPORTS = [
     "bkma_adv_batch_run_loop", "bkma_adv_batch_describe", "bkma_adv_batch_destroy",
     "bkma_conv1d_plan_create", "bkma_conv1d_plan_run", "bkma_conv1d_plan_describe", "bkma_conv1d_plan_destroy",
     "bkma_conv1d_plan_set_input_scale",
-    "bkma_conv1d_plan_set_input_scale",
-    "bkma_mfma_layout_probe", "bkma_mfma_fp8_layout_probe", "bkma_fp8_convert_probe",
 ]
 
 
@@ -190,9 +188,6 @@ def lib():
         "bkma_conv1d_plan_describe": (st, [vp, ctypes.c_char_p, u64]),
         "bkma_conv1d_plan_destroy": (None, [vp]),
         "bkma_conv1d_plan_set_input_scale": (st, [vp, ctypes.c_float]),
-        "bkma_mfma_layout_probe": (st, [vp, vp, vp, vp]),
-        "bkma_mfma_fp8_layout_probe": (st, [vp, vp, vp, vp]),
-        "bkma_fp8_convert_probe": (st, [vp, vp, u64, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -449,15 +444,3 @@ class Conv1dPlan:
             self.close()
         except Exception:
             pass
-
-
-def mfma_layout_probe(a, b, d, stream=None) -> None:
-    check(lib().bkma_mfma_layout_probe(_ptr(a), _ptr(b), _ptr(d), _stream(stream)))
-
-
-def mfma_fp8_layout_probe(a, b, d, stream=None) -> None:
-    check(lib().bkma_mfma_fp8_layout_probe(_ptr(a), _ptr(b), _ptr(d), _stream(stream)))
-
-
-def fp8_convert_probe(x, out, stream=None) -> None:
-    check(lib().bkma_fp8_convert_probe(_ptr(x), _ptr(out), x.numel(), _stream(stream)))

@@ -35,14 +35,17 @@ struct device_error : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
 
-inline void throw_on(bkma_status st) {
-    if (st == BKMA_OK) return;
-    const std::string msg = bkma_last_error();
+// A failing status as the reference's exception type.
+[[noreturn]] inline void throw_status(bkma_status st, const std::string &msg) {
     switch (st) {
         case BKMA_ERR_INVALID_ARGUMENT: throw std::invalid_argument(msg);
         case BKMA_ERR_DEVICE: throw device_error(msg);
         default: throw std::runtime_error(msg);
     }
+}
+
+inline void throw_on(bkma_status st) {
+    if (st != BKMA_OK) throw_status(st, bkma_last_error());
 }
 
 inline void hip_check(hipError_t e, const char *what) {
@@ -216,6 +219,9 @@ template <class T> bkma_span3d to_c(const span3d<T> &s) {
     return bkma_span3d{s.ptr, s.e[0], s.e[1], s.e[2]};
 }
 
+// The library's two solvers run through the C ABI (their kernels are in
+// libbkma_amd.so); any other solver runs through the header-only executors,
+// whose generic BkmaRunner is defined in bkma/plugin.hpp (hipcc only).
 template <class Solver, BkmaImpl Impl, class T = real_t> struct BkmaRunner;
 
 template <BkmaImpl Impl, class T> struct BkmaRunner<AdvectionSolver, Impl, T> {
@@ -259,12 +265,12 @@ inline std::string to_lowercase(const std::string &input) {
 template <class Solver, class T = real_t>
 using BkmaRunFn = std::function<Event(Queue &, span3d<T>, const Solver &, BkmaOptimParams, span3d<T>)>;
 
-// Accepts the three BkmaImpl names (the reference's switch lists NDRange and
-// AdaptiveWg, with BasicRange commented out because it needs a caller
-// scratch; here BasicRange is selectable and reports a missing scratch).
+// Same names, same lower-casing and same error as the reference: NDRange and
+// AdaptiveWg are selectable; BasicRange is rejected like any other name (its
+// case is commented out at impl_selector.hpp:38-39, since it needs a caller
+// scratch) and stays callable as bkma_run<Solver, BkmaImpl::BasicRange>.
 template <class Solver, class T = real_t> inline BkmaRunFn<Solver, T> impl_selector(const std::string &impl_name) {
     const std::string impl = to_lowercase(impl_name);
-    if (impl == "basicrange") return &bkma_run<Solver, BkmaImpl::BasicRange, T>;
     if (impl == "ndrange") return &bkma_run<Solver, BkmaImpl::NDRange, T>;
     if (impl == "adaptivewg") return &bkma_run<Solver, BkmaImpl::AdaptiveWg, T>;
     throw std::runtime_error(impl_name + " is not a valid implementation name.\n" + error_str);

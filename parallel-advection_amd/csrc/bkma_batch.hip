@@ -9,7 +9,7 @@
 // just before.  One step of the batch is
 //   batch_halo_kernel  (only if some problem is segmented)
 //   adv_batch_kernel   (persistent: work-groups pull items from a counter)
-// and `nsteps` steps are captured once into a hipGraph and replayed.
+// and runs of steps are captured into hipGraphs of up to 32 steps and replayed.
 //
 // The work counter is never reset: every execution of adv_batch_kernel hands
 // out exactly nitems valid tickets plus one terminating ticket per work-group,
@@ -297,6 +297,14 @@ extern "C" bkma_status bkma_adv_batch_create(const bkma_adv_problem *problems, i
         hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) != hipSuccess)
         return bail(fail(BKMA_ERR_DEVICE, "batch stream/event creation"));
 
+    // The timing runs below use the internal stream: order them after the
+    // caller's pending work on `stream` (its writes to the problems' arrays).
+    if (autotune) {
+        if (hipEventRecord(b->ev_in, as_stream(stream)) != hipSuccess ||
+            hipStreamWaitEvent(b->stream, b->ev_in, 0) != hipSuccess)
+            return bail(fail(BKMA_ERR_DEVICE, "batch: ordering after the caller's stream"));
+    }
+
     // Dynamic LDS budgets per work-group (the kernel adds a few static bytes).
     const u64 cap = lds_max - 1024;
     std::vector<u64> budgets = {std::min<u64>(65536, cap)};
@@ -339,8 +347,40 @@ extern "C" bkma_status bkma_adv_batch_create(const bkma_adv_problem *problems, i
     if (best_budget == 0) return bail(fail(BKMA_ERR_UNSUPPORTED, "advection batch: no LDS budget covers the problems"));
     st = build_items(*b, best_budget, best_threads);
     if (st != BKMA_OK) return bail(st);
-    (void)stream;
+    if (autotune) {  // later work on the caller's stream sees the timing runs' updates
+        if (hipEventRecord(b->ev_out, b->stream) != hipSuccess ||
+            hipStreamWaitEvent(as_stream(stream), b->ev_out, 0) != hipSuccess)
+            return bail(fail(BKMA_ERR_DEVICE, "batch: ordering the caller's stream after the timing runs"));
+    }
     *out = b;
+    return BKMA_OK;
+}
+
+// Time steps per captured graph: a run of n steps replays the graph of
+// kGraphSteps steps n / kGraphSteps times, then the graph of the remainder.
+constexpr u64 kGraphSteps = 32;
+
+// The instantiated graph of `nsteps` (<= kGraphSteps) batch steps, captured
+// on the batch's stream the first time it is asked for.
+static bkma_status batch_graph(bkma_adv_batch &b, u64 nsteps, hipGraphExec_t *out) {
+    auto it = b.graphs.find(nsteps);
+    if (it == b.graphs.end()) {
+        hipGraph_t g = nullptr;
+        BKMA_HIP_TRY(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+        bkma_status st = batch_steps(b, nsteps, b.stream);
+        hipError_t ce = hipStreamEndCapture(b.stream, &g);
+        if (st != BKMA_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+        hipGraphExec_t ge = nullptr;
+        hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return hip_fail(ie, "hipGraphInstantiate");
+        it = b.graphs.emplace(nsteps, ge).first;
+    }
+    *out = it->second;
     return BKMA_OK;
 }
 
@@ -352,24 +392,14 @@ extern "C" bkma_status bkma_adv_batch_run(bkma_adv_batch *b, uint64_t nsteps, in
     BKMA_HIP_TRY(hipEventRecord(b->ev_in, cs));
     BKMA_HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_in, 0));
     if (use_graph) {
-        auto it = b->graphs.find(nsteps);
-        if (it == b->graphs.end()) {
-            hipGraph_t g = nullptr;
-            BKMA_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            bkma_status st = batch_steps(*b, nsteps, b->stream);
-            hipError_t ce = hipStreamEndCapture(b->stream, &g);
-            if (st != BKMA_OK) {
-                if (g) (void)hipGraphDestroy(g);
-                return st;
-            }
-            if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
-            hipGraphExec_t ge = nullptr;
-            hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (ie != hipSuccess) return hip_fail(ie, "hipGraphInstantiate");
-            it = b->graphs.emplace(nsteps, ge).first;
-        }
-        BKMA_HIP_TRY(hipGraphLaunch(it->second, b->stream));
+        // Graphs of kGraphSteps steps and of the remainder: at most
+        // kGraphSteps graphs per batch, each captured on first use.
+        const u64 full = nsteps / kGraphSteps, rem = nsteps % kGraphSteps;
+        hipGraphExec_t gfull = nullptr, grem = nullptr;
+        if (full) BKMA_TRY(batch_graph(*b, kGraphSteps, &gfull));
+        if (rem) BKMA_TRY(batch_graph(*b, rem, &grem));
+        for (u64 i = 0; i < full; ++i) BKMA_HIP_TRY(hipGraphLaunch(gfull, b->stream));
+        if (rem) BKMA_HIP_TRY(hipGraphLaunch(grem, b->stream));
     } else {
         BKMA_TRY(batch_steps(*b, nsteps, b->stream));
     }

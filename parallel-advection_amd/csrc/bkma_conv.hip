@@ -32,18 +32,16 @@
 #include <vector>
 
 #include "common.hpp"
+#include "mfma_ops.hpp"
 
 namespace bkma {
 
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
 #ifndef BKMA_CONV_PAD
 #define BKMA_CONV_PAD 8
 #endif
 constexpr int kConvPad = BKMA_CONV_PAD;  // bf16 elements of padding per LDS row (keeps 16-byte alignment)
-typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ __bf16 to_bf16(float f) { return (__bf16)f; }  // round to nearest even
 template <class T> __device__ __forceinline__ float io_to_f32(T v) { return (float)v; }
 template <> __device__ __forceinline__ float io_to_f32<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
 template <class T> __device__ __forceinline__ T f32_to_io(float v) { return (T)v; }
@@ -581,33 +579,9 @@ __global__ __launch_bounds__(2 * C, 1) void conv_bf16x6_kernel(float *__restrict
     }
 }
 
-// Plain 32x32x16 bf16 MFMA over one fragment pair, for the layout test:
-// a[l][e], b[l][e] are the lane fragments, d[l][r] the accumulator.
-__global__ void mfma_layout_probe_kernel(const __bf16 *a, const __bf16 *b, float *d) {
-    const int l = threadIdx.x;
-    bf16x8_t av, bv;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        av[e] = a[l * 8 + e];
-        bv[e] = b[l * 8 + e];
-    }
-    f32x16_t c;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) c[r] = 0.0f;
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, c, 0, 0, 0);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) d[l * 16 + r] = c[r];
-}
-
-typedef int i32x8_t __attribute__((ext_vector_type(8)));
-typedef int i32x4_t __attribute__((ext_vector_type(4)));
-typedef int i32x2_t __attribute__((ext_vector_type(2)));
-
 // ---------------------------------------------------------------------------
 // FP8 (OCP e4m3) kernels
 // ---------------------------------------------------------------------------
-// Two FP32 values -> two e4m3 bytes (round to nearest even) in the low half.
-__device__ __forceinline__ int pack_fp8x2(float a, float b) { return __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false); }
 
 // Per-output-channel weight scale: the power of two 2^e >= amax/448 (so that
 // |w/scale| <= 448 and integers with few significant bits stay exact).
@@ -810,28 +784,6 @@ __global__ __launch_bounds__(2 * C, 1) void conv_fp8_kernel(TIO *__restrict__ x,
     }
 }
 
-
-// One FP8 (OCP e4m3) 32x32x64 MFMA with unit block scales (E8M0 127 = 2^0):
-// a[l][32], b[l][32] bytes are lane l's fragments, d[l][16] the result.
-__global__ void mfma_fp8_layout_probe_kernel(const i32x8_t *a, const i32x8_t *b, float *d) {
-    const int l = threadIdx.x;
-    f32x16_t c;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) c[r] = 0.0f;
-    c = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[l], b[l], c, 0, 0, 0, 127, 0, 127);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) d[l * 16 + r] = c[r];
-}
-
-// FP32 -> FP8 e4m3 with the conversion the FP8 kernels use (pairs, round to
-// nearest even, saturating).
-__global__ void fp8_convert_probe_kernel(const float *in, unsigned char *out, i64 npairs) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < npairs; i += (i64)gridDim.x * blockDim.x) {
-        const int packed = __builtin_amdgcn_cvt_pk_fp8_f32(in[2 * i], in[2 * i + 1], 0, false);
-        out[2 * i] = (unsigned char)(packed & 0xff);
-        out[2 * i + 1] = (unsigned char)((packed >> 8) & 0xff);
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Host side
@@ -1183,29 +1135,3 @@ extern "C" bkma_status bkma_conv1d_plan_set_input_scale(bkma_conv1d_plan *p, flo
 }
 
 extern "C" void bkma_conv1d_plan_destroy(bkma_conv1d_plan *p) { delete p; }
-
-extern "C" bkma_status bkma_mfma_fp8_layout_probe(const void *a, const void *b, void *d, void *stream) {
-    if (!a || !b || !d) return fail(BKMA_ERR_INVALID_ARGUMENT, "null argument");
-    BKMA_TRY(require_device());
-    hipLaunchKernelGGL(mfma_fp8_layout_probe_kernel, dim3(1), dim3(64), 0, as_stream(stream), (const i32x8_t *)a,
-                       (const i32x8_t *)b, (float *)d);
-    return check_launch("mfma_fp8_layout_probe_kernel");
-}
-
-extern "C" bkma_status bkma_fp8_convert_probe(const void *in, void *out, uint64_t n, void *stream) {
-    if (!in || !out || n % 2) return fail(BKMA_ERR_INVALID_ARGUMENT, "fp8 convert probe: null argument or odd n");
-    BKMA_TRY(require_device());
-    const i64 npairs = (i64)(n / 2);
-    const unsigned g = (unsigned)std::min<i64>(std::max<i64>(1, (npairs + 255) / 256), 4096);
-    hipLaunchKernelGGL(fp8_convert_probe_kernel, dim3(g), dim3(256), 0, as_stream(stream), (const float *)in,
-                       (unsigned char *)out, npairs);
-    return check_launch("fp8_convert_probe_kernel");
-}
-
-extern "C" bkma_status bkma_mfma_layout_probe(const void *a, const void *b, void *d, void *stream) {
-    if (!a || !b || !d) return fail(BKMA_ERR_INVALID_ARGUMENT, "null argument");
-    BKMA_TRY(require_device());
-    hipLaunchKernelGGL(mfma_layout_probe_kernel, dim3(1), dim3(64), 0, as_stream(stream), (const __bf16 *)a,
-                       (const __bf16 *)b, (float *)d);
-    return check_launch("mfma_layout_probe_kernel");
-}

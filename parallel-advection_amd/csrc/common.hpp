@@ -164,15 +164,8 @@ __device__ __forceinline__ i64 stencil_index_slow(i64 n1, int ipos1, int k) {
     return (i64)(((u64)n1 + (u64)(i64)ipos1 + (u64)k) % (u64)n1);
 }
 
-// Strided view of one (i0, :, i2) line.
-template <class T> struct LineRef {
-    T *base;
-    i64 stride;
-    __device__ __forceinline__ double operator()(i64 i) const { return to_f64(base[i * stride]); }
-    __device__ __forceinline__ T &at(i64 i) const { return base[i * stride]; }
-};
-
-// AdvectionSolver (src/solvers/AdvectionSolver.hpp:13-99) in FP64.
+// AdvectionSolver (src/solvers/AdvectionSolver.hpp:13-99) in FP64.  `line` is
+// an executor's line view (bkma/executors.hpp): line(i) is row i's value.
 struct AdvSolverDev {
     AdvDev p;
     __host__ __device__ __forceinline__ i64 window() const { return 1; }

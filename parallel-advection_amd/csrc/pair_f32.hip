@@ -1,18 +1,21 @@
 // Compiled column-pair variants for FP32 storage (the tuner's search space).
 // Each entry instantiates steps-per-pass 1 .. NSMAX of adv_column_pair_kernel
-// (adv_pair.hpp).
+// (adv_pair.hpp).  The DPP-broadcast variant is compiled in
+// pair_f32_dpp.hip, so the two build in parallel.
 #include "adv_pair.hpp"
 
 namespace bkma {
+
+const StreamVariant &pair_f32_dpp_variant();
 
 // (storage, stage-0 window, dtype, VEC, SMAX, P, waves/WG, non-temporal, DPP broadcast, NSMAX)
 const StreamVariant *pair_variants_f32(int *count) {
     static const StreamVariant table[] = {
 #ifdef BKMA_TUNED_FEW
-        BKMA_PAIR_VARIANT(float, double, BKMA_F32, 2, 3, 14, 1, true, true, 5),
+        pair_f32_dpp_variant(),
 #else
         BKMA_PAIR_VARIANT(float, double, BKMA_F32, 4, 3, 7, 1, true, false, 3),
-        BKMA_PAIR_VARIANT(float, double, BKMA_F32, 2, 3, 14, 1, true, true, 5),
+        pair_f32_dpp_variant(),
 #endif
     };
     *count = (int)(sizeof(table) / sizeof(table[0]));

@@ -38,3 +38,21 @@ int fac_adv_setup(const char *path, bkma_adv_params *out, char *impl, int len) {
     return c.ParseError();
 }
 }
+
+extern "C" {
+// Conv1dParamsNonCopyable::setup on a file: out = {length, channel_in,
+// channel_out, k, total_batch_size, batch_size_n2, n0, n1, n2, n_write, gpu,
+// pref_wg_size, seq_size0, seq_size2, inplace}.
+int fac_conv_setup(const char *path, unsigned long long out[15], char *impl, int len) {
+    ConfigMap c{std::string(path)};
+    bkma::Conv1dParamsNonCopyable p;
+    p.setup(c);
+    const unsigned long long v[15] = {p.length, p.channel_in, p.channel_out, p.k, p.total_batch_size,
+                                      p.batch_size_n2, p.n0, p.n1, p.n2, p.n_write, (unsigned long long)p.gpu,
+                                      p.pref_wg_size, p.seq_size0, p.seq_size2, (unsigned long long)p.inplace};
+    for (int i = 0; i < 15; ++i) out[i] = v[i];
+    std::strncpy(impl, p.kernelImpl.c_str(), (size_t)len - 1);
+    impl[len - 1] = '\0';
+    return c.ParseError();
+}
+}

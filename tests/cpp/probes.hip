@@ -1,0 +1,76 @@
+// Operand-layout and conversion probes of the MFMA Conv1d kernels: single
+// MFMAs on lane fragments and the kernels' FP32 -> FP8 e4m3 conversion
+// (parallel-advection_amd/csrc/mfma_ops.hpp).  TEST INFRASTRUCTURE: built into
+// tests/cpp/libbkma_probes.so, called by tests/test_conv_gpu.py.
+#include <algorithm>
+
+#include "mfma_ops.hpp"
+
+using namespace bkma;
+
+namespace {
+
+// One v_mfma_f32_32x32x16_bf16: a[l][e], b[l][e] are lane l's fragments, d[l][r] the accumulator.
+__global__ void mfma_bf16_kernel(const __bf16 *a, const __bf16 *b, float *d) {
+    const int l = threadIdx.x;
+    bf16x8_t av, bv;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        av[e] = a[l * 8 + e];
+        bv[e] = b[l * 8 + e];
+    }
+    f32x16_t c;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) c[r] = 0.0f;
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, c, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) d[l * 16 + r] = c[r];
+}
+
+// One v_mfma_scale_f32_32x32x64_f8f6f4 (e4m3 operands, unit block scales E8M0 127 = 2^0).
+__global__ void mfma_fp8_kernel(const i32x8_t *a, const i32x8_t *b, float *d) {
+    const int l = threadIdx.x;
+    f32x16_t c;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) c[r] = 0.0f;
+    c = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[l], b[l], c, 0, 0, 0, 127, 0, 127);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) d[l * 16 + r] = c[r];
+}
+
+__global__ void fp8_convert_kernel(const float *in, unsigned char *out, long long npairs) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < npairs;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int packed = pack_fp8x2(in[2 * i], in[2 * i + 1]);
+        out[2 * i] = (unsigned char)(packed & 0xff);
+        out[2 * i + 1] = (unsigned char)((packed >> 8) & 0xff);
+    }
+}
+
+int done(const char *) { return hipGetLastError() == hipSuccess ? 0 : 1; }
+
+}  // namespace
+
+extern "C" {
+// a[64][8], b[64][8] bf16 -> d[64][16] f32
+int probe_mfma_bf16(const void *a, const void *b, void *d, void *stream) {
+    hipLaunchKernelGGL(mfma_bf16_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, (const __bf16 *)a,
+                       (const __bf16 *)b, (float *)d);
+    return done("mfma_bf16_kernel");
+}
+// a[64][32], b[64][32] e4m3 bytes -> d[64][16] f32
+int probe_mfma_fp8(const void *a, const void *b, void *d, void *stream) {
+    hipLaunchKernelGGL(mfma_fp8_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, (const i32x8_t *)a,
+                       (const i32x8_t *)b, (float *)d);
+    return done("mfma_fp8_kernel");
+}
+// in[n] f32 (n even) -> out[n] e4m3 bytes
+int probe_fp8_convert(const void *in, void *out, unsigned long long n, void *stream) {
+    if (n % 2) return 1;
+    const long long np = (long long)(n / 2);
+    const unsigned g = (unsigned)std::min<long long>((np + 255) / 256, 4096);
+    hipLaunchKernelGGL(fp8_convert_kernel, dim3(g ? g : 1), dim3(256), 0, (hipStream_t)stream, (const float *)in,
+                       (unsigned char *)out, np);
+    return done("fp8_convert_kernel");
+}
+}

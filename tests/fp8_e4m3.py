@@ -43,3 +43,10 @@ def encode(x: np.ndarray) -> np.ndarray:
     code = np.where(a >= 448.0, 0x7E, code)
     sign = np.signbit(x).astype(np.uint8) << 7
     return (code.astype(np.uint8) | sign).astype(np.uint8)
+
+
+def encode_saturating(x: np.ndarray) -> np.ndarray:
+    """encode() after saturating |x| > 448 (infinities included) to +-448, the conversion of the FP8 Conv1d
+    kernels; NaN is left to the caller."""
+    x = np.asarray(x, dtype=np.float64)
+    return encode(np.where(np.abs(x) > 448.0, np.copysign(448.0, x), x))

@@ -126,3 +126,80 @@ def test_adv_setup_from_reference_ini(readers):
     assert p.dt == float(np.float32(0.001)) and p.minRealVx == -1.0 and p.maxRealVx == 1.0
     assert p.pref_wg_size == 512 and p.gpu == 1 and impl.value == b"AdaptiveWg"
     assert p.dx == 1 / 1024 and p.dvx == 2 / 1024 and p.inv_dx == 1024
+
+
+CONV_FIELDS = ("length", "channel_in", "channel_out", "k", "total_batch_size", "batch_size_n2", "n0", "n1", "n2",
+               "n_write", "gpu", "pref_wg_size", "seq_size0", "seq_size2", "inplace")
+
+
+def _conv_setup(libs, path):
+    fac = ctypes.CDLL(libs["fac"]._name)
+    fac.fac_conv_setup.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_char_p,
+                                   ctypes.c_int]
+    out = (ctypes.c_ulonglong * 15)()
+    impl = ctypes.create_string_buffer(64)
+    err = fac.fac_conv_setup(path.encode(), out, impl, 64)
+    return err, dict(zip(CONV_FIELDS, list(out))), impl.value
+
+
+def _ref_reads(libs, path):
+    """The raw values Conv1dParamsNonCopyable::setup (src/config/Conv1dParams.cpp:44-68) reads, through the
+    reference's own reader, with the reference's defaults."""
+    L = libs["ref"]
+    h = L.ref_ini_open(path.encode())
+    gi = lambda s, k, d: L.ref_ini_get_integer(h, s.encode(), k.encode(), d)  # noqa: E731
+    gb = lambda s, k, d: L.ref_ini_get_bool(h, s.encode(), k.encode(), d)  # noqa: E731
+    buf = ctypes.create_string_buffer(64)
+    L.ref_ini_get_string(h, b"impl", b"kernelImpl", b"AdaptiveWg", buf, 64)
+    r = {"length": gi("problem", "length", 1024), "channel_in": gi("problem", "channel_in", 3),
+         "k": gi("problem", "k", 3), "total_batch_size": gi("problem", "total_batch_size", 262144),
+         "batch_size_n2": gi("problem", "batch_size_n2", 512), "inplace": gb("impl", "inplace", 1),
+         "gpu": gb("optimization", "gpu", 1), "pref_wg_size": gi("optimization", "pref_wg_size", 512),
+         "seq_size0": gi("optimization", "seq_size0", 1), "seq_size2": gi("optimization", "seq_size2", 1),
+         "kernelImpl": buf.value}
+    L.ref_ini_close(h)
+    return r
+
+
+def test_conv_setup_from_reference_ini(readers):
+    # the reference's conv1d.ini: every derived size is a known answer of Conv1dParams.cpp:44-68
+    libs, files = readers
+    if "ref_conv1d.ini" not in files:
+        pytest.skip("reference conv1d.ini absent")
+    err, p, impl = _conv_setup(libs, files["ref_conv1d.ini"])
+    assert err == 0 and impl == b"AdaptiveWg"
+    assert p == {"length": 1024, "channel_in": 3, "channel_out": 3, "k": 3, "total_batch_size": 262144,
+                 "batch_size_n2": 512, "n0": 512, "n1": 3072, "n2": 512, "n_write": 1022, "gpu": 1,
+                 "pref_wg_size": 512, "seq_size0": 1, "seq_size2": 1, "inplace": 1}
+    ref = _ref_reads(libs, files["ref_conv1d.ini"])
+    for k in ("length", "channel_in", "k", "total_batch_size", "batch_size_n2", "pref_wg_size", "seq_size0",
+              "seq_size2", "gpu", "inplace"):
+        assert p[k] == ref[k], k
+
+
+def test_conv_setup_short_fields_wrap_like_the_reference(readers, tmp_path):
+    # channel_in, k and pref_wg_size are `short unsigned` in Conv1dParams (Conv1dParams.hpp:16-31), so values
+    # read as long wrap modulo 2^16: 65539 -> 3, 65541 -> 5, 66048 -> 512.
+    libs, _ = readers
+    f = tmp_path / "wrap.ini"
+    f.write_text("[problem]\nlength = 100\nchannel_in = 65539\nk = 65541\ntotal_batch_size = 96\n"
+                 "batch_size_n2 = 8\n[impl]\nkernelImpl = NDRange\ninplace = false\n"
+                 "[optimization]\ngpu = no\npref_wg_size = 66048\nseq_size0 = 2\nseq_size2 = 4\n")
+    err, p, impl = _conv_setup(libs, str(f))
+    ref = _ref_reads(libs, str(f))
+    assert err == 0 and impl == b"NDRange" == ref["kernelImpl"]
+    assert (ref["channel_in"], ref["k"], ref["pref_wg_size"]) == (65539, 65541, 66048)
+    assert (p["channel_in"], p["channel_out"], p["k"], p["pref_wg_size"]) == (3, 3, 5, 512)
+    assert (p["n0"], p["n1"], p["n2"], p["n_write"]) == (12, 300, 8, 96)
+    assert (p["gpu"], p["inplace"], p["seq_size0"], p["seq_size2"]) == (0, 0, 2, 4)
+
+
+def test_conv_setup_defaults(readers, tmp_path):
+    libs, _ = readers
+    f = tmp_path / "empty.ini"
+    f.write_text("[problem]\n")
+    err, p, impl = _conv_setup(libs, str(f))
+    assert err == 0 and impl == b"AdaptiveWg"
+    assert p == {"length": 1024, "channel_in": 3, "channel_out": 3, "k": 3, "total_batch_size": 262144,
+                 "batch_size_n2": 512, "n0": 512, "n1": 3072, "n2": 512, "n_write": 1022, "gpu": 1,
+                 "pref_wg_size": 512, "seq_size0": 1, "seq_size2": 1, "inplace": 1}

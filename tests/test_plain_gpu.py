@@ -114,6 +114,24 @@ def test_adaptivewg_multi_batch(cuda):
     _assert_bit_exact(got, ref)
 
 
+def test_adaptivewg_batches_not_multiple_of_grid_stride(cuda):
+    # Batches of 5 lines in dims 0 and 2 with w0 = w2 = 2 give g = 2 work-groups, a grid stride of 4 lines:
+    # one work-item's second trip lies past its batch.  Every line is still stepped exactly once per call.
+    n0, n1, n2 = 10, 256, 10
+    p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    po = O.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
+    host = O.uniform(n0 * n1 * n2, 13).reshape(n0, n1, n2)
+    o = B.create_optim_params(n0, n1, n2, pref_wg_size=64, elem_size=8)
+    o.dispatch_d0 = B.lib().bkma_init_1d_blocking(n0, 5)
+    o.dispatch_d2 = B.lib().bkma_init_1d_blocking(n2, 5)
+    o.w0, o.w2 = 2, 2
+    assert (o.dispatch_d0.n_batch, o.dispatch_d0.batch_size) == (2, 5)
+    got = _run_gpu(host, B.ADAPTIVE_WG, p, 3, o)
+    ref = host.copy()
+    O.advect_f64(ref, po, 3)
+    _assert_bit_exact(got, ref)
+
+
 @pytest.mark.parametrize("shape", [(64, 1024, 1), (7, 333, 5)])
 def test_ndrange(cuda, shape):
     n0, n1, n2 = shape

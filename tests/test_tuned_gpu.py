@@ -283,23 +283,60 @@ def test_autotune_picks_a_correct_variant(cuda):
     assert np.array_equal(got, ref)
 
 
+def _c1_near_identity_params(first=992, n0=64):
+    """The strided benchmark's line shape [.][1024][512] and its own slabs first .. first+n0-1, whose
+    velocities v = -1 + i0/1024 surround v = 0 (slab 1024): shifts v*dt/dx of a few thousandths of a cell,
+    where each step changes a value by less than half an FP32 ulp and per-step rounding loses the most.  A CPU
+    scan of every slab with the oracle puts the workload's largest FP32 errors at slabs 1023-1026 and
+    their mirrors around slabs 24 and 2024 (shifts of about -1 and +1 cells).  v0 and dvx = 1/1024 are exact,
+    so these are the benchmark's own velocities."""
+    p = B.make_params(n0, 1024, 512, dt=1e-3, minRealVx=-1 + first / 1024, maxRealVx=-1 + (first + n0) / 1024)
+    assert p.dvx == 1 / 1024 and p.minRealVx == -1 + first / 1024
+    return p
+
+
 def test_f32_error_budget_vs_f64(cuda):
-    # 50 steps of the strided benchmark shape, reduced in n0.  One FP32
-    # rounding per step (<= 1/2 ulp, ulp = 2^-23 for the overshoot values in
-    # [1, 2)) accumulates on near-identity lines: 50 * 2^-24 / max|f| bounds the
-    # coherent worst case; the measured value is printed.
+    # The autotuned FP32 plan, all 50 steps in one call (its passes keep the intermediate steps in FP64 and
+    # round once per pass): rel_max <= 1e-6 against the FP64 plan (bit-identical to the reference's FP64).
     torch = _torch()
-    n0, n1, n2 = 64, 1024, 512
-    p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
-    d64 = torch.empty((n0, n1, n2), dtype=torch.float64, device="cuda")
+    p = _c1_near_identity_params()
+    d64 = torch.empty((p.n0, p.n1, p.n2), dtype=torch.float64, device="cuda")
     B.fill_uniform(d64, 1)
     d32 = d64.to(torch.float32)
+    plan = B.AdvPlan(p, B.F32, autotune=True)
+    cfg, name = plan.describe()
     B.AdvPlan(p, B.F64).run(d64, 50)
-    B.AdvPlan(p, B.F32).run(d32, 50)
+    plan.run(d32, 50)
     rel_max, rel_l2 = B.rel_errors(d32, d64)
-    print(f"FP32 vs FP64 after 50 steps: rel_max {rel_max:.3e} rel_L2 {rel_l2:.3e}", flush=True)
-    assert rel_max <= 50 * 2.0 ** -24
+    print(f"{name}: FP32 vs FP64 after 50 steps in one call: rel_max {rel_max:.3e} rel_L2 {rel_l2:.3e}", flush=True)
+    assert rel_max <= 1e-6
     assert rel_l2 <= 3e-7
+
+
+def test_f32_error_of_one_call_per_step(cuda):
+    # The reference's time loop calls bkma_run once per step (src/advection.cpp:55-58).  With FP32 storage
+    # every call ends in one FP32 rounding, so 50 calls round 50 times: on these near-identity slabs, where
+    # updates below half an ulp are lost at every rounding, the error grows by up to 2^-24 (half an ulp of the
+    # values in [1, 2) that the interpolation overshoot reaches) per step: 50 * 2^-24 = 2.98e-6 relative to
+    # max|f| >= 1.  The oracle gives 1.44e-6 on these slabs (1.8e-6 over the full workload), above the 1e-6
+    # that the plan meets with whole runs per call; the results stay bit-identical to the oracle's
+    # one-rounding-per-step restatement.
+    torch = _torch()
+    p = _c1_near_identity_params(1016, 16)
+    d64 = torch.empty((p.n0, p.n1, p.n2), dtype=torch.float64, device="cuda")
+    B.fill_uniform(d64, 1)
+    d32 = d64.to(torch.float32)
+    host = d32.cpu().numpy()
+    plan = B.AdvPlan(p, B.F32, autotune=True)
+    B.AdvPlan(p, B.F64).run(d64, 50)
+    for _ in range(50):
+        plan.run(d32, 1)
+    rel_max, rel_l2 = B.rel_errors(d32, d64)
+    print(f"FP32, 50 calls of one step: rel_max {rel_max:.3e} rel_L2 {rel_l2:.3e}", flush=True)
+    assert rel_max <= 50 * 2.0 ** -24
+    po = O.make_params(p.n0, p.n1, p.n2, dt=1e-3, minRealVx=p.minRealVx, maxRealVx=p.maxRealVx)
+    O.advect_f32_blocked(host, po, 50, 1)
+    assert np.array_equal(d32.cpu().numpy().view(np.uint32), host.view(np.uint32))
 
 
 def test_steps_per_pass_beyond_compiled_is_rejected(cuda):
