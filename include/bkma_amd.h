@@ -152,6 +152,19 @@ bkma_status bkma_scratch_extents(bkma_impl impl, const bkma_optim_params *optim,
 bkma_status bkma_adv_run(bkma_impl impl, bkma_dtype dtype, void *data, const bkma_adv_params *params,
                          const bkma_optim_params *optim, bkma_span3d scratch, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): bkma_adv_run with the global scratch of AdaptiveWg/Global and
+ * BasicRange taken from a library-owned device array of
+ * bkma_static_scratch_bytes() bytes (64 MiB) per device, held until the
+ * process ends, instead of a caller span.  Calls no hipMalloc/hipFree.
+ * A shape whose scratch exceeds the array is rejected with
+ * BKMA_ERR_INVALID_ARGUMENT.  Every call on one device shares that device's
+ * array: calls that overlap on one GPU (several streams or host threads)
+ * race.  Local/NDRange calls need no scratch and behave as bkma_adv_run. */
+bkma_status bkma_adv_run_noalloc(bkma_impl impl, bkma_dtype dtype, void *data, const bkma_adv_params *params,
+                                 const bkma_optim_params *optim, void *stream);
+uint64_t bkma_static_scratch_bytes(void);
 
 /* ------------------------------------------------------------------ */
 /* bkma_run<ConvSolver, Impl>  (src/core/bkma_run.hpp:10-62)           */

@@ -133,7 +133,7 @@ EXPORTS = [
     "bkma_adv_plan_destroy", "bkma_adv_plan_variants", "bkma_adv_batch_create", "bkma_adv_batch_run",
     "bkma_adv_batch_run_loop", "bkma_adv_batch_describe", "bkma_adv_batch_destroy",
     "bkma_conv1d_plan_create", "bkma_conv1d_plan_run", "bkma_conv1d_plan_describe", "bkma_conv1d_plan_destroy",
-    "bkma_conv1d_plan_set_input_scale",
+    "bkma_conv1d_plan_set_input_scale", "bkma_adv_run_noalloc", "bkma_static_scratch_bytes",
 ]
 
 
@@ -161,6 +161,8 @@ def lib():
         "bkma_init_1d_blocking": (BatchConfig1D, [u64, u64]),
         "bkma_scratch_extents": (st, [ctypes.c_int, P(OptimParams), u64, u64, u64, u64, P(u64), P(u64), P(u64)]),
         "bkma_adv_run": (st, [ctypes.c_int, ctypes.c_int, vp, P(AdvParams), P(OptimParams), Span3d, vp]),
+        "bkma_adv_run_noalloc": (st, [ctypes.c_int, ctypes.c_int, vp, P(AdvParams), P(OptimParams), vp]),
+        "bkma_static_scratch_bytes": (u64, []),
         "bkma_conv_run": (st, [ctypes.c_int, ctypes.c_int, vp, u64, u64, u64, vp, vp, u64, u64, u64,
                                P(OptimParams), Span3d, vp]),
         "bkma_fill_uniform": (st, [ctypes.c_int, vp, u64, u64, vp]),
@@ -270,6 +272,12 @@ def adv_run(impl, data, params, optim=None, scratch=None, scratch_extents=None, 
     check(lib().bkma_adv_run(impl, dtype_code(data), _ptr(data), ctypes.byref(params),
                              ctypes.byref(optim) if optim is not None else None,
                              _span(scratch, scratch_extents), _stream(stream)))
+
+
+def adv_run_noalloc(impl, data, params, optim=None, stream=None) -> None:
+    """bkma_adv_run_noalloc: one in-place step; any global scratch comes from the library's static array."""
+    check(lib().bkma_adv_run_noalloc(impl, dtype_code(data), _ptr(data), ctypes.byref(params),
+                                     ctypes.byref(optim) if optim is not None else None, _stream(stream)))
 
 
 def conv_run(impl, data, weight, bias, K, C, L, optim=None, scratch=None, scratch_extents=None,
