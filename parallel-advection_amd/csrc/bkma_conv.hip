@@ -21,9 +21,14 @@
 //    owns 32 output channels and keeps its W fragments in registers for the
 //    whole kernel (weight stationary).  NPROD = 1: bf16 x bf16; NPROD = 3:
 //    split precision (x = xh + xl, w = wh + wl, y += xh wh + xh wl + xl wh).
+//  - conv_tap_bf16_kernel: the BF16 mode with BF16 storage.  One B fragment
+//    of X feeds the K taps' MFMAs (K accumulators over the same columns,
+//    combined by DPP lane shifts), so it reads K times fewer LDS fragments
+//    than conv_mfma_kernel; its loads and stores are whole 64-byte pieces.
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -374,6 +379,351 @@ __global__ __launch_bounds__(2 * C, 1) void conv_mfma_kernel(TIO *__restrict__ x
         __syncthreads();
         tile = next;
     }
+}
+
+// Staging task u of conv_tap_bf16_kernel loads in block 2u (at most TB-3)
+// and is written to LDS two blocks later.
+template <int TB> constexpr int tap_ld_blk(int u) { return 2 * u < TB - 3 ? 2 * u : TB - 3; }
+
+// Tap-reuse implicit GEMM for BF16 storage on v_mfma_f32_32x32x16_bf16 (FP32
+// accumulation), for shapes whose weight fragments stay in registers
+// (K*C/4 per lane).  Per tap k, D_k = W_k X with A = W_k (the wave's 32
+// output channels x 16 input channels, packed as in conv_mfma_kernel and kept
+// in AGPRs for the whole kernel) and B = X (16 input channels x 32 columns,
+// one ds_read_b128 of the channel-last LDS tile).  The B fragment of a k-step
+// is shared by the K taps' MFMAs, which accumulate into K accumulators over
+// the same 32 unshifted columns: a block of 32 columns reads C/16 fragments
+// for K*C/16 MFMAs.  Lane l of every D_k holds column c0 + l%32.  The output
+//   y[oc][c0 + j] = bias[oc] + sum_k D_k[oc][c0 + j + k],   j < OB = 33 - K,
+// is combined in place by Horner's rule, y = D_0 + S(D_1 + S(D_2 + ...)),
+// where S moves every column down by one lane: one DPP add (wave_shl:1, lane
+// l reads lane l+1) per register and level.  Lanes 31 and 63 read values of
+// another column set, which only reach the columns j >= OB that are not
+// stored (consecutive blocks overlap by K - 1 columns), so no partial sum
+// crosses a block.  The bias is added before the conversion to bf16.
+//
+// Schedule.  With one wave per SIMD the wave overlaps its own VALU and memory
+// work with its MFMAs (an MFMA occupies the matrix core for 32 cycles, in
+// which the wave can issue about six independent VALU instructions).  The
+// blocks of a tile are unrolled; after each MFMA of block b comes one slice
+// of the other work: block b-1's combination and stores (the other
+// accumulator set), the B-fragment read two k-steps ahead, and the staging
+// of the next tile into the other LDS buffer.  Scheduling barriers keep that
+// order.  Block b-1's accumulators are read by VALU instructions only after
+// an s_nop fence of 20 wait states that follows block b's first MFMA.  The
+// last block's combination runs in the next tile's first block.
+//
+// Memory.  Global loads and stores are buffer instructions with one
+// descriptor per batch element.  Tiles start at multiples of 64 bytes of a
+// row, so the staging loads read whole 64-byte pieces, and the outputs go
+// through a per-wave LDS ring so that they are stored as whole 64-byte
+// pieces too (fewer, full requests to the L2).  A store that must not be
+// written (past L-K+1) gets an offset beyond the descriptor's range, which
+// the hardware drops: the block code has no branches.
+//
+// LDS.  A row holds C bf16 and 32 bytes of padding (18 chunks of 16 bytes for
+// C = 128, so row r starts at bank group 2r mod 8), and its channel chunks
+// are XOR-swizzled by row bits 2..5.  A B-fragment read (8 lanes: 8
+// consecutive rows, one chunk) and a staging write (8 lanes: rows 4q+c over 8
+// consecutive q, one chunk) then both hit 8 different groups of 4 banks.
+// Requirements (checked by the host): L % 8 == 0 and C*L*2 < 2^31.
+template <int C, int K, int TB>
+__global__ __launch_bounds__(2 * C, 1) void conv_tap_bf16_kernel(__hip_bfloat16 *__restrict__ x,
+                                                               const __hip_bfloat16 *__restrict__ halo,
+                                                               const __bf16 *__restrict__ wq,
+                                                               const float *__restrict__ bias, int L, int tiles,
+                                                               int total_tiles, int chunk) {
+    constexpr int NW = C / 32;
+    constexpr int NT = 64 * NW;
+    constexpr int KS = C / 16;       // k-steps (16 channels) per tap
+    constexpr int OB = 33 - K;       // output columns per 32-column block
+    constexpr int TO = TB * OB;      // output columns per tile
+    constexpr int NXC = TO + K - 1;  // staged input columns per tile
+    constexpr int RSB = 2 * C + 32;  // bytes per LDS row
+    constexpr int BUFB = NXC * RSB;  // bytes per LDS buffer
+    constexpr int NQ = NXC / 4;      // 4-column groups per tile
+    constexpr int NTASK = (C / 8) * NQ;         // staging tasks (8 channels x 4 columns)
+    constexpr int TPT = (NTASK + NT - 1) / NT;  // staging tasks per thread (the last may repeat one)
+    static_assert(C % 128 == 0, "at least 16 chunks of 16 bytes per LDS row (the swizzle takes 4 bits)");
+    static_assert(K >= 2 && (K - 1) % 4 == 0 && TO % 4 == 0, "halo and tile columns are whole quads");
+    static_assert(TB % 2 == 0, "the two accumulator sets alternate over the blocks of a tile");
+    static_assert(K + 3 <= KS, "k-steps: addressing, K-1 combination levels, stores, staging store and loads");
+    static_assert(TPT >= 1 && (TPT < 2 || 2 * (TPT - 2) < TB - 3) && tap_ld_blk<TB>(TPT - 1) + 2 < TB,
+                  "staging tasks load at blocks 0, 2, 4, ... (the last at TB-3 at most) and store two blocks later");
+
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int n = lane & 31, h = lane >> 5;
+    const int Lout = L - K + 1;
+    const unsigned num_records = (unsigned)(C * L * 2);
+    constexpr unsigned kDrop = 0x80000000u;  // store offset beyond every descriptor's range
+
+    int tile = blockIdx.x * chunk;
+    const int tile_end = min(tile + chunk, total_tiles);
+    if (tile >= tile_end) return;  // empty chunk (uniform over the work-group)
+
+    bf16x8_t wf[K * KS];  // tap k, k-step s at k * KS + s (the packed order r = k*C + ic)
+    {
+        const bf16x8_t *src = reinterpret_cast<const bf16x8_t *>(wq) + ((i64)wv * K * KS) * 64 + lane;
+#pragma unroll
+        for (int i = 0; i < K * KS; ++i) wf[i] = src[i * 64];
+    }
+    // D lane l, register r: output channel 32*wv + 8*(r/4) + 4*h + r%4.
+    float bias_r[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) bias_r[r] = bias[32 * wv + 8 * (r / 4) + 4 * h + (r % 4)];
+
+    // Staging: thread tid moves channels 8g .. 8g+7 (g = tid / 16) of the
+    // 4-column groups q = tid % 16 + 16u, u < TPT (16 lanes read 128
+    // contiguous bytes of a row).  Groups past the tile repeat its last one.
+    // Row q of the swizzle is q % 16 for every u, so the LDS offset of task u
+    // is st_lds + u * 64 * RSB.
+    static_assert(NT == 256 && C / 8 == 16 && 16 * (TPT - 1) < NQ && NQ <= 16 * TPT,
+                  "staging map: 16 channel groups x 16 column groups per pass");
+    const int sg = tid >> 4, sq = tid & 15;
+    const int st_row = 8 * sg * L * 2;
+    const int st_lds = 4 * sq * RSB + 16 * (sg ^ sq);
+    const int q_last = min(sq + 16 * (TPT - 1), NQ - 1);
+    const int st_lds_last = 4 * q_last * RSB + 16 * (sg ^ (q_last & 15));
+    // Output staging.  Tiles are whole 32-column sectors (64 bytes of a row),
+    // and a wave stores each sector of its 32 channel rows once it is
+    // complete, as 64-byte row pieces.  The combined block (lane n holding
+    // column n of channels 8*(r/4) + 4*h + r%4) is written as bf16 to a
+    // per-wave ring of 64 columns x 32 channel rows in LDS (after the two
+    // input buffers, OUTRS bytes per row); block p completes sector p-1 of
+    // its tile (28(p+1) >= 32p for p <= 7), which is read back with lane l
+    // holding channel l/4 + 16i, columns 8*(l%4) .. +7 of the sector, and
+    // stored with one 16-byte store per lane and i (an 8-byte store covers
+    // the 4 columns before L-K+1 when a piece straddles it).
+    static_assert(TO % 32 == 0 && TO / 32 == TB - 1 && OB >= 16, "a tile is TB-1 sectors; block p completes sector p-1");
+    constexpr int OUTRS = 192;  // bytes per ring row: 128 + 64 (rows 1 apart in different bank halves)
+    const int out_base = 2 * BUFB + wv * 32 * OUTRS;
+    const int sc_wr = out_base + 4 * h * OUTRS;  // + 2 * ((c0 + n) & 63) + m * OUTRS
+    const int sc_rd = out_base + (lane >> 2) * OUTRS + 16 * (lane & 3);  // + i*16*OUTRS + 2*(32s & 63)
+    const unsigned st_lane = (unsigned)(((32 * wv + (lane >> 2)) * L + 8 * (lane & 3)) * 2);
+    auto rsrc_of = [&](int tl) __attribute__((always_inline)) {
+        const int b = tl / tiles;
+        return __builtin_amdgcn_make_buffer_rsrc(x + (i64)b * C * L, 0, (int)num_records, 0x00020000);
+    };
+    // Task u of a tile at column tcol: 8 channels x 4 columns as 8 row quads.
+    auto stage_load = [&](__amdgpu_buffer_rsrc_t rs, int tcol, int u, u32x2_t(&dst)[8]) __attribute__((always_inline)) {
+        // columns past the row end are clamped into it: they only feed masked outputs
+        const int col = u == TPT - 1 ? 4 * q_last : 4 * sq + 64 * u;
+        const unsigned voff = (unsigned)(st_row + 2 * min(tcol + col, L - 4));
+#pragma unroll
+        for (int i = 0; i < 8; ++i) dst[i] = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, i * L * 2, 0);
+    };
+    // Column c of task u: dword d holds channels 2d (low half) and 2d+1.
+    auto stage_store_col = [&](unsigned char *buf, int u, int c, const u32x2_t(&src)[8]) __attribute__((always_inline)) {
+        u32x4_t v;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const unsigned lo = src[2 * d][c >> 1], hi = src[2 * d + 1][c >> 1];
+            v[d] = ((c & 1) ? ((lo >> 16) | (hi & 0xFFFF0000u)) : ((lo & 0xFFFFu) | (hi << 16)));
+        }
+        *reinterpret_cast<u32x4_t *>(buf + (u == TPT - 1 ? st_lds_last : st_lds + u * 64 * RSB) + c * RSB) = v;
+    };
+    // Registers r, r+1 of the combined block at local column c0, bias added,
+    // to the ring (channel rows 8*(r/4) + r%4 and the next one of this lane's
+    // group).  Lanes n >= OB write columns of the next block, which that
+    // block overwrites.
+    auto ring_pair = [&](const f32x16_t &y, int r, int c0) __attribute__((always_inline)) {
+        const unsigned p = pack_bf16x2(y[r] + bias_r[r], y[r + 1] + bias_r[r + 1]);
+        const int m = 8 * (r / 4) + (r % 4);
+        unsigned char *dst = lds_raw + sc_wr + 2 * ((c0 + n) & 63);
+        *reinterpret_cast<unsigned short *>(dst + m * OUTRS) = (unsigned short)p;
+        *reinterpret_cast<unsigned short *>(dst + (m + 1) * OUTRS) = (unsigned short)(p >> 16);
+    };
+    auto ring_read = [&](int sec, int i) __attribute__((always_inline)) {
+        return *reinterpret_cast<const u32x4_t *>(lds_raw + sc_rd + i * 16 * OUTRS + 2 * ((32 * sec) & 63));
+    };
+    // Half i of sector sec of the tile at column tcol (lim = L-K+1 - tcol):
+    // a lane's 8 columns when all lie below lim, else their first 4 when
+    // those do; the other stores are dropped.
+    auto store_sector = [&](const u32x4_t &v, int sec, int i, __amdgpu_buffer_rsrc_t rs, int tcol, int lim)
+                            __attribute__((always_inline)) {
+        const int c = 32 * sec + 8 * (lane & 3);
+        const unsigned off = st_lane + (unsigned)((i * 16 * L + tcol + 32 * sec) * 2);
+        __builtin_amdgcn_raw_buffer_store_b128(v, rs, (int)(c + 8 <= lim ? off : kDrop), 0, 0);
+        const u32x2_t lo = {v[0], v[1]};
+        __builtin_amdgcn_raw_buffer_store_b64(lo, rs, (int)(c + 8 > lim && c + 4 <= lim ? off : kDrop), 0, 0);
+    };
+    // y[kk] += S(y[kk + 1]) for registers r0 .. r0+NR-1 (DPP wave_shl:1).
+    auto horner = [&](f32x16_t &yk, const f32x16_t &y1, int r0, int nr) __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = r0; r < r0 + nr; ++r) {
+            const float v = y1[r];
+            const float sv = __builtin_bit_cast(
+                float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+            yk[r] += sv;
+        }
+    };
+
+    // Initial tile: staged synchronously into buffer 0.
+    {
+        const __amdgpu_buffer_rsrc_t rs = rsrc_of(tile);
+        const int tcol = (tile % tiles) * TO;
+#pragma unroll
+        for (int u = 0; u < TPT; ++u) {
+            u32x2_t ld[8];
+            stage_load(rs, tcol, u, ld);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) stage_store_col(lds_raw, u, c, ld);
+        }
+    }
+    // Every load issued so far (weights, bias, the first tile) has arrived:
+    // without this wait the loop would wait for them in every tile, and with
+    // them for all the stores of the tile before.
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    __syncthreads();
+
+    f32x16_t acc0[K], acc1[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc1[k][r] = 0.0f;
+    // The combination of the previous block may belong to the previous tile:
+    // its descriptor and column base.  Nothing is stored before the first block.
+    __amdgpu_buffer_rsrc_t rs_prev = rsrc_of(tile);
+    int tcol_prev = 0, lim_prev = INT_MIN;
+
+    for (int it = 0; tile < tile_end; ++it) {
+        const int t = tile % tiles;
+        unsigned char *const cur = lds_raw + (it & 1) * BUFB;
+        unsigned char *const nxt = lds_raw + ((it + 1) & 1) * BUFB;
+        if (tile == tile_end - 1 && t < tiles - 1) {
+            // The chunk's last tile: its K-1 halo columns come from the
+            // snapshot (the next chunk may already have overwritten them).
+            const __hip_bfloat16 *hs = halo + (i64)blockIdx.x * C * (K - 1);
+            for (int e = tid; e < C * (K - 1); e += NT) {
+                const int ic = e / (K - 1), row = TO + e % (K - 1);
+                *reinterpret_cast<__hip_bfloat16 *>(cur + row * RSB + 16 * ((ic >> 3) ^ ((row >> 2) & 15)) +
+                                                    2 * (ic & 7)) = hs[e];
+            }
+            __syncthreads();
+        }
+        // The last tile of a chunk stages itself again into the unused
+        // buffer, so that the block code has no branch around the staging.
+        const int stage_tl = tile + 1 < tile_end ? tile + 1 : tile;
+        const __amdgpu_buffer_rsrc_t rs_ld = rsrc_of(stage_tl);
+        const int tcol_ld = (stage_tl % tiles) * TO;
+        const __amdgpu_buffer_rsrc_t rs_st = rsrc_of(tile);
+        const int tcol = t * TO;
+        const int lim = Lout - tcol;  // block c0 stores local columns c0 + n < min(OB, lim - c0)
+
+        // B-fragment addressing of block blk: row c0 + n, chunk (2s + h) ^ M(row).
+        // The lane's column is re-read per tile (an opaque copy), so that the
+        // compiler computes these addresses per block instead of holding all
+        // of them in registers across the loop.
+        int n_t = n;
+        asm volatile("" : "+v"(n_t));
+        auto b_addr = [&](int blk, int &base, int &sw) __attribute__((always_inline)) {
+            const int row = blk * OB + n_t;
+            const int m = (row >> 2) & 15;
+            base = (it & 1) * BUFB + row * RSB + 16 * (h ^ (m & 1));
+            sw = 32 * (m >> 1);
+        };
+        auto b_read = [&](int base, int sw, int s) __attribute__((always_inline)) {
+            return *reinterpret_cast<const bf16x8_t *>(lds_raw + base + ((32 * s) ^ sw));
+        };
+        int b_base, b_sw, nb_base = 0, nb_sw = 0;
+        b_addr(0, b_base, b_sw);
+        bf16x8_t fr[3];
+        fr[0] = b_read(b_base, b_sw, 0);
+        fr[1] = b_read(b_base, b_sw, 1);
+        u32x2_t ld[2][8];
+
+        static_for<0, TB>([&](auto blk_) __attribute__((always_inline)) {
+            constexpr int BLK = decltype(blk_)::value;
+            f32x16_t(&cur_acc)[K] = (BLK & 1) ? acc1 : acc0;
+            f32x16_t(&prv)[K] = (BLK & 1) ? acc0 : acc1;
+            // the previous block: this tile's block BLK-1, or the previous tile's last
+            constexpr int c0p = BLK ? (BLK - 1) * OB : (TB - 1) * OB;
+            const __amdgpu_buffer_rsrc_t rsp = BLK ? rs_st : rs_prev;
+            const int tcp = BLK ? tcol : tcol_prev;
+            const int limp = BLK ? lim : lim_prev;
+            u32x4_t sec_v[2];
+            static_for<0, KS>([&](auto s_) __attribute__((always_inline)) {
+                constexpr int S = decltype(s_)::value;
+                constexpr int F = BLK * KS + S;  // fragment index within the tile
+                // read two k-steps ahead (the next block's first two in this block's last two)
+                if constexpr (F + 2 < TB * KS) {
+                    if constexpr (S + 2 < KS)
+                        fr[(F + 2) % 3] = b_read(b_base, b_sw, S + 2);
+                    else
+                        fr[(F + 2) % 3] = b_read(nb_base, nb_sw, S + 2 - KS);
+                }
+                const bf16x8_t bx = fr[F % 3];
+                static_for<0, K>([&](auto k_) __attribute__((always_inline)) {
+                    constexpr int KK = decltype(k_)::value;
+                    if constexpr (S == 0)
+                        asm volatile("v_mfma_f32_32x32x16_bf16 %0, %1, %2, 0"
+                                     : "=&v"(cur_acc[KK])
+                                     : "a"(wf[KK * KS]), "v"(bx));
+                    else
+                        asm volatile("v_mfma_f32_32x32x16_bf16 %0, %1, %2, %0"
+                                     : "+v"(cur_acc[KK])
+                                     : "a"(wf[KK * KS + S]), "v"(bx));
+                    // ---- one slice of the other work ----
+                    if constexpr (S == 0 && KK == 0) {
+                        // fence: the previous block's accumulators are read by VALU below
+                        asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 3" : "+v"(prv[0]), "+v"(prv[1]));
+#pragma unroll
+                        for (int q = 2; q < K; ++q) asm volatile("" : "+v"(prv[q]));
+                    } else if constexpr (S == 0 && KK == 1) {
+                        if constexpr (BLK + 1 < TB) b_addr(BLK + 1, nb_base, nb_sw);
+                    } else if constexpr (S >= 1 && S <= K - 1) {
+                        // Horner level: prv[kk] += S(prv[kk + 1]), four registers per MFMA
+                        constexpr int KKL = K - 1 - S;
+                        if constexpr (KK < 4) horner(prv[KKL], prv[KKL + 1], 4 * KK, 4);
+                    } else if constexpr (S == K) {
+                        constexpr int PB = BLK ? BLK - 1 : TB - 1;  // the previous block's index in its tile
+                        if constexpr (KK < 4) {
+                            ring_pair(prv[0], 4 * KK, c0p);
+                            ring_pair(prv[0], 4 * KK + 2, c0p);
+                        } else if constexpr (PB >= 1) {
+                            sec_v[0] = ring_read(PB - 1, 0);
+                            sec_v[1] = ring_read(PB - 1, 1);
+                        }
+                    } else if constexpr (S == K + 1) {
+                        // staging store of the task loaded two blocks ago
+                        static_for<0, TPT>([&](auto u_) __attribute__((always_inline)) {
+                            constexpr int U = decltype(u_)::value;
+                            if constexpr (tap_ld_blk<TB>(U) + 2 == BLK && KK < 4) stage_store_col(nxt, U, KK, ld[U & 1]);
+                        });
+                    } else if constexpr (S == K + 2) {
+                        static_for<0, TPT>([&](auto u_) __attribute__((always_inline)) {
+                            constexpr int U = decltype(u_)::value;
+                            if constexpr (tap_ld_blk<TB>(U) == BLK && KK == 0) stage_load(rs_ld, tcol_ld, U, ld[U & 1]);
+                        });
+                        constexpr int PB = BLK ? BLK - 1 : TB - 1;
+                        if constexpr (PB >= 1 && KK >= 1 && KK <= 2) store_sector(sec_v[KK - 1], PB - 1, KK - 1, rsp, tcp, limp);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                });
+            });
+            if constexpr (BLK + 1 < TB) {
+                b_base = nb_base;
+                b_sw = nb_sw;
+            }
+        });
+        __syncthreads();
+        rs_prev = rs_st;
+        tcol_prev = tcol;
+        lim_prev = lim;
+        tile = tile + 1;
+    }
+    // The last block's combination and stores (accumulator set 1: TB is even).
+    asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 3" : "+v"(acc1[0]), "+v"(acc1[1]));
+#pragma unroll
+    for (int k = 2; k < K; ++k) asm volatile("" : "+v"(acc1[k]));
+#pragma unroll
+    for (int kk = K - 2; kk >= 0; --kk) horner(acc1[kk], acc1[kk + 1], 0, 16);
+#pragma unroll
+    for (int r = 0; r < 16; r += 2) ring_pair(acc1[0], r, (TB - 1) * OB);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) store_sector(ring_read(TB - 2, i), TB - 2, i, rs_prev, tcol_prev, lim_prev);
 }
 
 // Three-term split W fragments: w = w0 + w1 + w2 exactly (each bf16 piece
@@ -841,21 +1191,50 @@ template <class TIO, int NPROD, int C, int K, int TN> int occupancy_mfma(u64 lds
     return blocks;
 }
 
+// Tap-reuse kernel (BF16 storage): TB blocks of 33-K output columns per tile.
+template <int C, int K, int TB> void launch_tap(void *x, const ConvPlan &p, hipStream_t s) {
+    constexpr int TO = TB * (33 - K);
+    const int tiles = (int)ceil_div((u64)(p.L - p.K + 1), (u64)TO);
+    const int total = (int)(p.B * tiles);
+    int grid = 0, chunk = 0;
+    launch_halo<__hip_bfloat16>(x, p, C, K, TO, tiles, total, &grid, &chunk, s);
+    auto k = conv_tap_bf16_kernel<C, K, TB>;
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds_bytes);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(2 * C), (size_t)p.lds_bytes, s, reinterpret_cast<__hip_bfloat16 *>(x),
+                       reinterpret_cast<const __hip_bfloat16 *>(p.halo), (const __bf16 *)p.w, (const float *)p.bias,
+                       (int)p.L, tiles, total, chunk);
+}
+
+template <int C, int K, int TB> int occupancy_tap(u64 lds_bytes) {
+    auto k = conv_tap_bf16_kernel<C, K, TB>;
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 2 * C, (size_t)lds_bytes) != hipSuccess) return 0;
+    return blocks;
+}
+
+// TN: output columns per tile; tap: the tap-reuse kernel (TN = TB * (33 - K)).
 struct MfmaVariant {
     bkma_dtype io;
     int nprod, C, K, TN;
     MfmaLaunch fn;
     int (*occ)(u64 lds_bytes);
+    int tap = 0;
 };
 
-#define MV(TIO, IO, NP, C, K, TN) {IO, NP, C, K, TN, &launch_mfma<TIO, NP, C, K, TN>, &occupancy_mfma<TIO, NP, C, K, TN>}
+#define MV(TIO, IO, NP, C, K, TN) {IO, NP, C, K, TN, &launch_mfma<TIO, NP, C, K, TN>, &occupancy_mfma<TIO, NP, C, K, TN>, 0}
+#define TV(C, K, TB) {BKMA_BF16, 1, C, K, TB * (33 - K), &launch_tap<C, K, TB>, &occupancy_tap<C, K, TB>, 1}
+// The first entry that matches is taken; the tap kernel needs L % 8 == 0 and
+// C*L*2 < 2^31, the entries after it cover the other lengths.
 const MfmaVariant kMfmaVariants[] = {
-    MV(float, BKMA_F32, 1, 128, 5, 128),           MV(float, BKMA_F32, 3, 128, 5, 64),
+    TV(128, 5, 8),
     MV(__hip_bfloat16, BKMA_BF16, 1, 128, 5, 128), MV(__hip_bfloat16, BKMA_BF16, 3, 128, 5, 64),
-    MV(float, BKMA_F32, 1, 128, 3, 128),           MV(float, BKMA_F32, 3, 128, 3, 64),
-    MV(float, BKMA_F32, 1, 64, 5, 128),            MV(float, BKMA_F32, 3, 64, 5, 64),
+    MV(float, BKMA_F32, 1, 128, 5, 128),           MV(float, BKMA_F32, 3, 128, 5, 64),
+    MV(float, BKMA_F32, 1, 128, 3, 128),      MV(float, BKMA_F32, 3, 128, 3, 64),
+    MV(float, BKMA_F32, 1, 64, 5, 128),       MV(float, BKMA_F32, 3, 64, 5, 64),
 };
 #undef MV
+#undef TV
 
 // Grid (one work-group per chunk of consecutive tiles) and the halo copies
 // of the chunks' last tiles.
@@ -938,9 +1317,11 @@ static const MfmaVariant *find_fp8(bkma_dtype io, i64 C, i64 K) {
     return nullptr;
 }
 
-static const MfmaVariant *find_mfma(bkma_dtype io, int nprod, i64 C, i64 K) {
+static const MfmaVariant *find_mfma(bkma_dtype io, int nprod, i64 C, i64 K, i64 L) {
     for (const auto &v : kMfmaVariants)
-        if (v.io == io && v.nprod == nprod && v.C == C && v.K == K) return &v;
+        if (v.io == io && v.nprod == nprod && v.C == C && v.K == K &&
+            (!v.tap || (L % 8 == 0 && C * L * 2 < (i64(1) << 31))))
+            return &v;
     return nullptr;
 }
 
@@ -990,7 +1371,7 @@ extern "C" bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype i
         p->name = want == BKMA_F64 ? "conv_direct<f64>" : "conv_direct<f32>";
     } else if (mode == BKMA_CONV_BF16 || mode == BKMA_CONV_BF16X3) {
         const int nprod = mode == BKMA_CONV_BF16 ? 1 : 3;
-        const MfmaVariant *mv = find_mfma(io_dtype, nprod, (i64)C, (i64)K);
+        const MfmaVariant *mv = find_mfma(io_dtype, nprod, (i64)C, (i64)K, (i64)L);
         if (!mv)
             return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: no MFMA kernel compiled for C=" + std::to_string(C) +
                                                        ", K=" + std::to_string(K) + " and this io dtype"));
@@ -1004,7 +1385,8 @@ extern "C" bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype i
         if (hipMemcpyAsync(p->bias, bias, C * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
             return bail(fail(BKMA_ERR_DEVICE, "conv1d plan: bias copy"));
         const bool xlo = nprod == 3 && io_dtype != BKMA_BF16;
-        p->lds_bytes = 2ull * (u64)(mv->TN + K - 1) * (u64)(C + kConvPad) * 2 * (xlo ? 2 : 1);
+        p->lds_bytes = mv->tap ? 2ull * (u64)(mv->TN + K - 1) * (u64)(2 * C + 32) + (u64)(C / 32) * 32 * 192
+                               : 2ull * (u64)(mv->TN + K - 1) * (u64)(C + kConvPad) * 2 * (xlo ? 2 : 1);
         if (p->lds_bytes > lds_max) return bail(fail(BKMA_ERR_UNSUPPORTED, "conv1d plan: LDS tile too large"));
         int dev = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
@@ -1014,9 +1396,9 @@ extern "C" bkma_status bkma_conv1d_plan_create(bkma_conv_mode mode, bkma_dtype i
         const u64 esz = io_dtype == BKMA_BF16 ? 2 : 4;
         if (hipMalloc(&p->halo, std::max<u64>((u64)p->resident_groups * C * (K - 1) * esz, 16)) != hipSuccess)
             return bail(fail(BKMA_ERR_OUT_OF_MEMORY, "conv1d plan: halo buffer"));
-        p->name = std::string("conv_mfma_bf16") + (nprod == 3 ? "x3" : "") + "<C" + std::to_string(C) + ",K" +
-                  std::to_string(K) + ",TN" + std::to_string(mv->TN) + (io_dtype == BKMA_BF16 ? ",io_bf16" : ",io_f32") +
-                  ">";
+        p->name = std::string(mv->tap ? "conv_tap_bf16" : "conv_mfma_bf16") + (nprod == 3 ? "x3" : "") + "<C" +
+                  std::to_string(C) + ",K" + std::to_string(K) + ",TN" + std::to_string(mv->TN) +
+                  (io_dtype == BKMA_BF16 ? ",io_bf16" : ",io_f32") + ">";
     } else if (mode == BKMA_CONV_BF16X6) {
         const MfmaVariant *mv = find_bf16x6(io_dtype, (i64)C, (i64)K);
         if (!mv)
@@ -1095,7 +1477,7 @@ extern "C" bkma_status bkma_conv1d_plan_run(bkma_conv1d_plan *p, void *x, void *
         }
         case BKMA_CONV_BF16:
         case BKMA_CONV_BF16X3: {
-            const MfmaVariant *v = find_mfma(p->io, p->mode == BKMA_CONV_BF16 ? 1 : 3, p->C, p->K);
+            const MfmaVariant *v = find_mfma(p->io, p->mode == BKMA_CONV_BF16 ? 1 : 3, p->C, p->K, p->L);
             if (!v) return fail(BKMA_ERR_UNSUPPORTED, "conv1d run: no MFMA kernel");
             v->fn(x, *p, s);
             break;

@@ -11,8 +11,19 @@ typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef int i32x8_t __attribute__((ext_vector_type(8)));
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
 typedef int i32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ __bf16 to_bf16(float f) { return (__bf16)f; }  // round to nearest even
+
+// Two FP32 values -> two bf16 (round to nearest even), a in the low half:
+// one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ unsigned pack_bf16x2(float a, float b) {
+    typedef float f32x2_v __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2_v __attribute__((ext_vector_type(2)));
+    const f32x2_v v = {a, b};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2_v));
+}
 
 // Largest finite OCP FP8 e4m3 value.
 constexpr float kFp8E4m3Max = 448.0f;

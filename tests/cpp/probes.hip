@@ -47,6 +47,12 @@ __global__ void fp8_convert_kernel(const float *in, unsigned char *out, long lon
     }
 }
 
+// DPP wave_shl:1 (dpp_ctrl 0x130) of a, with 0 for lanes that have no source.
+__global__ void dpp_wave_shl_kernel(const unsigned *a, unsigned *out) {
+    const int l = threadIdx.x;
+    out[l] = (unsigned)__builtin_amdgcn_update_dpp(0, (int)a[l], 0x130, 0xf, 0xf, false);
+}
+
 int done(const char *) { return hipGetLastError() == hipSuccess ? 0 : 1; }
 
 }  // namespace
@@ -63,6 +69,12 @@ int probe_mfma_fp8(const void *a, const void *b, void *d, void *stream) {
     hipLaunchKernelGGL(mfma_fp8_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, (const i32x8_t *)a,
                        (const i32x8_t *)b, (float *)d);
     return done("mfma_fp8_kernel");
+}
+// a[64] u32 -> out[64]: the lane shift of DPP wave_shl:1
+int probe_dpp_wave_shl(const void *a, void *out, void *stream) {
+    hipLaunchKernelGGL(dpp_wave_shl_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, (const unsigned *)a,
+                       (unsigned *)out);
+    return done("dpp_wave_shl_kernel");
 }
 // in[n] f32 (n even) -> out[n] e4m3 bytes
 int probe_fp8_convert(const void *in, void *out, unsigned long long n, void *stream) {

@@ -64,6 +64,21 @@ def test_mfma_32x32x16_bf16_layout(cuda):
     assert np.array_equal(got, D)
 
 
+def test_dpp_wave_shl_lane_layout(cuda):
+    # The tap-reuse conv kernel moves every column of an accumulator down by one lane with DPP
+    # wave_shl:1 (dpp_ctrl 0x130): lane l reads lane l+1 across the whole wave, and lane 63, which has
+    # no source, gets the old value (0 here).
+    torch = _torch()
+    a = torch.arange(100, 164, dtype=torch.int32, device="cuda")
+    out = torch.empty(64, dtype=torch.int32, device="cuda")
+    _probe("probe_dpp_wave_shl", a, out)
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    an = a.cpu().numpy()
+    assert np.array_equal(o[:63], an[1:])
+    assert o[63] == 0
+
+
 def _case(Bn, C, L, K, seed):
     rng = np.random.default_rng(seed)
     x = rng.standard_normal((Bn, C, L))
@@ -345,6 +360,32 @@ def test_conv_bf16_storage_many_tiles(cuda, L):
     Lout = L - K + 1
     rmax, rl2 = _rel(got, ref, Lout)
     print(plan.name(), f"rel_max {rmax:.3e} rel_L2 {rl2:.3e}", flush=True)
+    assert rmax <= 1e-2
+    assert np.array_equal(got[:, :, Lout:], x_in[:, :, Lout:])
+
+
+def test_conv_bf16_storage_long_chunks(cuda):
+    """BF16 storage at the full line length: 96 x 15 tiles of 280 columns over
+    the resident work-groups give chunks of several tiles that cross batch
+    elements, so the cross-tile pipeline (a tile's last block is combined in
+    the next tile's first block) and the snapshots at chunk ends all run.
+    Sampled batch elements are checked against the FP64 oracle."""
+    torch = _torch()
+    Bn, C, L, K = 96, 128, 4096, 5
+    x, w, b = _case(Bn, C, L, K, 17)
+    tx = torch.from_numpy(x).to(torch.bfloat16).cuda()
+    x_in = tx.double().cpu().numpy()
+    plan = B.Conv1dPlan(B.CONV_BF16, B.BF16, Bn, C, L, K, torch.from_numpy(w).float().cuda(),
+                        torch.from_numpy(b).float().cuda())
+    print(plan.name(), flush=True)
+    plan.run(tx)
+    torch.cuda.synchronize()
+    got = tx.double().cpu().numpy()
+    Lout = L - K + 1
+    idx = [0, 1, 2, 31, 47, 48, 63, 94, 95]
+    ref = _ref(x_in[idx], w.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64))
+    rmax, rl2 = _rel(got[idx], ref, Lout)
+    print(f"rel_max {rmax:.3e} rel_L2 {rl2:.3e}", flush=True)
     assert rmax <= 1e-2
     assert np.array_equal(got[:, :, Lout:], x_in[:, :, Lout:])
 
