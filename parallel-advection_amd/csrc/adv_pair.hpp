@@ -155,6 +155,7 @@ __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs
     const i64 n1 = p.n1, n2 = p.n2;
     const i64 col = (tile * 64 + lane) * VEC;
     const bool lane_ok = col < n2;  // VEC divides n2: the whole vector is in range
+    const bool full_tile = wave_uniform((int)((tile * 64 + 64) * VEC <= n2)) != 0;
     T *slab = data + (i0 * n1) * n2 + (lane_ok ? col : 0);
     const i64 first_end = R + n1;  // end of the positions read from HBM
     const i64 total = first_end + H;
@@ -180,6 +181,10 @@ __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs
             for (int v = 0; v < VEC; ++v) dst[v] = (T)0;
         }
     };
+    // A steady trip: every lane in range, no position of the trip in the
+    // first or last trips' special cases (rows read before row 0, head rows,
+    // outputs before the first stored row, positions past the line).
+    auto load_row = [&](i64 pos, T (&dst)[VEC]) __attribute__((always_inline)) { M::load(slab + (pos - R) * n2, dst); };
     static_for<0, P>([&](auto sc) __attribute__((always_inline)) {
         constexpr int s = decltype(sc)::value;
         load_pos(s, stg[s]);
@@ -189,7 +194,8 @@ __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs
     double tab[kPairTaps];
     double tset[NSET][kPairTaps];
 
-    for (i64 base = 0; base < total; base += U) {
+    i64 base = 0;
+    auto trip = [&](auto steady_c) __attribute__((always_inline)) {
         // Foot table: lane l holds the K coefficients of the row at position
         // base - TB + l.
         {
@@ -218,22 +224,30 @@ __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs
                 for (int k = 0; k < K; ++k) tset[s][k] = lane_gather(tab[k], 16 * s + (lane & 15));
         }
 
-        auto step = [&](auto jc) __attribute__((always_inline)) {
+        auto step = [&](auto jc, auto steady_c) __attribute__((always_inline)) {
             constexpr int J = decltype(jc)::value;
+            constexpr bool STEADY = decltype(steady_c)::value;
             const i64 pin = base + J;
-            if (pin >= total) return;
+            if constexpr (!STEADY) {
+                if (pin >= total) return;
+            }
             // 1. position pin enters the stage-0 window
             {
                 constexpr int sw = J % W, ss = J % P;
-                if (pin < first_end) {
+                if (STEADY || pin < first_end) {
 #pragma unroll
                     for (int v = 0; v < VEC; ++v) w0[sw][v] = (WT)stg[ss][v];
-                    if (pin >= R && pin < R + H) {
+                    if constexpr (!STEADY) {
+                        if (pin >= R && pin < R + H) {
 #pragma unroll
-                        for (int v = 0; v < VEC; ++v) head[pin - R][lane * VEC + v] = stg[ss][v];
+                            for (int v = 0; v < VEC; ++v) head[pin - R][lane * VEC + v] = stg[ss][v];
+                        }
                     }
                     // 2. refill the staging slot with position pin + P
-                    if (pin + P < first_end) load_pos(pin + P, stg[ss]);
+                    if constexpr (STEADY)
+                        load_row(pin + P, stg[ss]);
+                    else if (pin + P < first_end)
+                        load_pos(pin + P, stg[ss]);
                 } else {
                     const int h = (int)(pin - first_end);
 #pragma unroll
@@ -275,12 +289,20 @@ __device__ __forceinline__ void pair_unit(T *__restrict__ data, const StreamArgs
                     else
                         taps(wm[NS - 2], o);
                     const i64 pt = pin - NS * AHEAD;
-                    if (pt >= R && pt < first_end && lane_ok) M::store(slab + (pt - R) * n2, o);
+                    if (STEADY || (pt >= R && pt < first_end && lane_ok)) M::store(slab + (pt - R) * n2, o);
                 }
             });
         };
-        static_for<0, U>(step);
-    }
+        static_for<0, U>([&](auto jc) __attribute__((always_inline)) { step(jc, steady_c); });
+    };
+    // Trips are taken in order; the steady ones (bases s_lo .. s_hi, all
+    // multiples of U) run a body without the range checks, in a loop of its
+    // own (one loop with both bodies needs far more registers).
+    const i64 s_lo = full_tile ? (R + H + U - 1) / U * U : total;
+    const i64 s_hi = full_tile ? first_end - U - P : -1;
+    for (; base < total && base < s_lo; base += U) trip(std::false_type{});
+    for (; base <= s_hi; base += U) trip(std::true_type{});
+    for (; base < total; base += U) trip(std::false_type{});
 }
 
 // Selects the K-tap instance of D (wave-uniform) in [DLO, DHI].
@@ -302,7 +324,7 @@ __device__ __forceinline__ void pair_dispatch(int d, T *__restrict__ data, const
 // NS * a.pair_ahead rows per wave (pair_ahead = 3 - the smallest shift of the
 // problem, the largest stage lag of its slabs).
 template <class T, class WT, int VEC, int SMAX, int P, int NS, int WPB, bool NT, bool DPP>
-__global__ __launch_bounds__(64 * WPB) void adv_column_pair_kernel(T *__restrict__ data, StreamArgs a) {
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) void adv_column_pair_kernel(T *__restrict__ data, StreamArgs a) {
     static_assert(SMAX <= 3, "stage lag 3 - D must stay non-negative");
     extern __shared__ __align__(16) unsigned char pair_lds[];
     const int lane = threadIdx.x & 63;
