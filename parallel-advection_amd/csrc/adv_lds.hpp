@@ -249,22 +249,129 @@ __device__ __forceinline__ void line_step(const S *__restrict__ src, D *__restri
     }
 }
 
+// The feet of a thread's rows, kept across the fused steps of a pass (the
+// foot of row i1 of slab i0 is the same at every step): for each of its at
+// most RMAX rows, the first stencil row and the six coefficients.  line_step
+// computes the same values at every step; for contiguous lines (n2 = 1) that
+// FP64 work dominates the step, one foot per cell.
+template <int RMAX> struct LineFeet {
+    i64 base[RMAX];
+    double coef[RMAX][6];
+};
+
+// Thread layout of line_step for VC columns per thread.
+struct LineLayout {
+    int cw, rows_per_pass, t_row, t_col;
+};
+template <int VC> __device__ __forceinline__ LineLayout line_layout(const AdvDev &p) {
+    LineLayout l;
+    const i64 nch = p.n2 / VC;
+    l.cw = (int)min<i64>(nch, blockDim.x);
+    l.rows_per_pass = blockDim.x / l.cw;
+    l.t_row = threadIdx.x / l.cw;
+    l.t_col = threadIdx.x - l.t_row * l.cw;
+    return l;
+}
+
+template <int VC, int RMAX>
+__device__ __forceinline__ void line_feet(const AdvDev &p, i64 i0, i64 nslab, LineFeet<RMAX> &ft) {
+    const LineLayout l = line_layout<VC>(p);
+    const i64 n1 = p.n1;
+#pragma unroll
+    for (int j = 0; j < RMAX; ++j) {
+        const i64 rr = l.t_row + (i64)j * l.rows_per_pass;
+        ft.base[j] = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ft.coef[j][k] = 0.;
+        if (l.t_row < l.rows_per_pass && rr < nslab * n1) {
+            const i64 g = rr / n1, i1 = rr - g * n1;
+            double d_prev1;
+            const int leftNode = adv_foot(p, (int)(i0 + g), (int)i1, d_prev1);
+            adv_lag_basis(d_prev1, ft.coef[j]);
+            const int ipos1 = leftNode - 2;
+            ft.base[j] = stencil_fast_ok(n1, ipos1) ? stencil_base(n1, ipos1) : stencil_index_slow(n1, ipos1, 0);
+        }
+    }
+}
+
+// line_step with the feet of LineFeet (every row of the unit is one of the
+// thread's at most RMAX rows).
+template <class T, int VC, int RMAX, class S, class D>
+__device__ __forceinline__ void line_step_feet(const S *__restrict__ src, D *__restrict__ dst, const AdvDev &p,
+                                               i64 nslab, const LineFeet<RMAX> &ft) {
+    const LineLayout l = line_layout<VC>(p);
+    const i64 n1 = p.n1, n2 = p.n2, nch = n2 / VC;
+    if (l.t_row >= l.rows_per_pass) return;
+#pragma unroll
+    for (int j = 0; j < RMAX; ++j) {
+        const i64 rr = l.t_row + (i64)j * l.rows_per_pass;
+        if (rr >= nslab * n1) break;
+        const i64 g = rr / n1, i1 = rr - g * n1;
+        const S *gsrc = src + g * n1 * n2;
+        D *gdst = dst + g * n1 * n2;
+        for (i64 ch = l.t_col; ch < nch; ch += l.cw) {
+            double acc[VC];
+#pragma unroll
+            for (int v = 0; v < VC; ++v) acc[v] = 0.;
+            i64 r = ft.base[j];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+#pragma unroll
+                for (int v = 0; v < VC; ++v) {
+                    const double f = (double)gsrc[r * n2 + ch * VC + v];
+                    if constexpr (sizeof(T) == 8) acc[v] = acc[v] + ft.coef[j][k] * f;
+                    else acc[v] = fma(ft.coef[j][k], f, acc[v]);
+                }
+                r += 1;
+                if (r >= n1) r -= n1;
+            }
+#pragma unroll
+            for (int v = 0; v < VC; ++v) gdst[i1 * n2 + ch * VC + v] = (D)acc[v];
+        }
+    }
+}
+
+// Rows per thread whose feet a fused pass keeps in registers (about 40
+// VGPRs per row): none for 16-wave work-groups, whose waves have 128 VGPRs.
+template <int NT> constexpr int line_feet_rows() { return NT <= 256 ? 4 : (NT <= 512 ? 2 : 0); }
+
 // Whole lines of the nslab consecutive slabs i0 .. advanced by ns steps in
 // one pass: staged once into `lds` (storage type), steps 1 .. ns-1 into the
-// FP64 buffers w0, w1 in turn, step ns from LDS to the slabs in HBM.  Every
-// thread calls it; it has no early return.
-template <class T>
+// FP64 buffers w0, w1 in turn, step ns from LDS to the slabs in HBM.  When
+// every thread has at most RMAX rows, the feet are computed once for the ns
+// steps.  Every thread calls it; it has no early return.
+template <class T, int RMAX>
 __device__ __forceinline__ void line_steps_unit(T *__restrict__ slab, const AdvDev &p, i64 i0, i64 nslab, int ns,
                                                 T *__restrict__ lds, double *__restrict__ w0,
                                                 double *__restrict__ w1) {
     const i64 cnt = nslab * p.n1 * p.n2;
     for (i64 e = threadIdx.x; e < cnt; e += blockDim.x) lds[e] = slab[e];
+    const bool even = p.n2 % 2 == 0;
+    constexpr int RM = RMAX > 0 ? RMAX : 1;
+    LineFeet<RM> ft;
+    bool keep = false;  // uniform
+    if constexpr (RMAX > 0) {
+        const LineLayout lay = even ? line_layout<2>(p) : line_layout<1>(p);
+        keep = ns > 1 && nslab * p.n1 <= (i64)RMAX * lay.rows_per_pass;
+        if (keep) {
+            if (even)
+                line_feet<2>(p, i0, nslab, ft);
+            else
+                line_feet<1>(p, i0, nslab, ft);
+        }
+    }
     __syncthreads();
     auto step = [&](const auto *src, auto *dst) __attribute__((always_inline)) {
-        if (p.n2 % 2 == 0)
+        if (RMAX > 0 && keep) {
+            if (even)
+                line_step_feet<T, 2>(src, dst, p, nslab, ft);
+            else
+                line_step_feet<T, 1>(src, dst, p, nslab, ft);
+        } else if (even) {
             line_step<T, 2>(src, dst, p, i0, nslab);
-        else
+        } else {
             line_step<T, 1>(src, dst, p, i0, nslab);
+        }
         __syncthreads();
     };
     if (ns == 1) {
@@ -295,7 +402,7 @@ __global__ __launch_bounds__(NT) void adv_line_lds_kernel(T *__restrict__ data, 
         for (i64 unit = blockIdx.x; unit < units; unit += gridDim.x) {
             const i64 i0 = unit * a.slabs;
             const i64 nslab = min(a.slabs, a.p.n0 - i0);
-            line_steps_unit<T>(data + i0 * per, a.p, i0, nslab, (int)a.ns, st, w0, w0 + cnt);
+            line_steps_unit<T, line_feet_rows<NT>()>(data + i0 * per, a.p, i0, nslab, (int)a.ns, st, w0, w0 + cnt);
             __syncthreads();  // the next unit restages the LDS
         }
         return;
