@@ -257,12 +257,14 @@ int32_t bkma_adv_plan_variants(bkma_dtype dtype, bkma_tuned_config *out, int32_t
 /* ------------------------------------------------------------------ */
 /* Many independent problems (each its own [n0][n1][n2] array, dt and
  * velocity grid; the reference runs one bkma_run per problem and step)
- * advanced together: every step is one persistent launch over all
- * problems' work items (plus a halo-snapshot launch when a problem's lines
- * exceed the LDS budget), and bkma_adv_batch_run(use_graph = 1) captures the
- * nsteps steps into one hipGraph, replayed on later calls with the same
- * nsteps.  Results are bit-identical to running each problem's steps alone
- * (FP64: the reference's arithmetic; FP32: the tuned FP32 arithmetic). */
+ * advanced together: every pass of ns steps (the batch's steps per pass,
+ * 1 .. 4) is one persistent launch over all problems' work items (plus a
+ * halo-snapshot launch when a problem's lines exceed the LDS budget), and
+ * bkma_adv_batch_run(use_graph = 1) captures the passes into hipGraphs,
+ * replayed on later calls.  A run of nsteps steps makes nsteps / ns passes of
+ * ns steps, then one pass of the rest.  Results are bit-identical to running
+ * each problem's passes alone (FP64: the reference's arithmetic, whatever
+ * ns; FP32: the tuned FP32 arithmetic with one rounding to FP32 per pass). */
 typedef struct bkma_adv_problem {
     void *data;              /* device array [n0][n1][n2] of the batch dtype, 16-byte aligned */
     bkma_adv_params params;  /* update_deltas applied */
@@ -270,11 +272,18 @@ typedef struct bkma_adv_problem {
 
 typedef struct bkma_adv_batch bkma_adv_batch;
 
-/* autotune = 1 times the LDS budgets (32 KB .. the device limit) on the
- * problems' own arrays, which the timing steps modify: create the batch
- * before initialising the data. */
+/* autotune = 1 times the LDS budgets (32 KB .. the device limit), work-group
+ * sizes and steps per pass (1 .. 4) on the problems' own arrays, which the
+ * timing steps modify: create the batch before initialising the data.
+ * autotune = 0: 64 KB, 256 threads, one step per pass. */
 bkma_status bkma_adv_batch_create(const bkma_adv_problem *problems, int32_t nproblems, bkma_dtype dtype,
                                   int32_t autotune, void *stream, bkma_adv_batch **out);
+/* A batch with an explicit configuration: LDS budget per work-group (bytes),
+ * threads per work-group (256 or 1024; steps_per_pass > 1 needs 256) and
+ * steps per pass (1 .. 4).  BKMA_ERR_UNSUPPORTED when a problem does not fit. */
+bkma_status bkma_adv_batch_create_with(const bkma_adv_problem *problems, int32_t nproblems, bkma_dtype dtype,
+                                       uint64_t lds_budget, int32_t threads, int32_t steps_per_pass, void *stream,
+                                       bkma_adv_batch **out);
 bkma_status bkma_adv_batch_run(bkma_adv_batch *batch, uint64_t nsteps, int32_t use_graph, void *stream);
 /* Baselines over the same problems: mode 0 = the reference's loop of
  * per-problem bkma_run<AdvectionSolver, AdaptiveWg> launches with
@@ -282,6 +291,8 @@ bkma_status bkma_adv_batch_run(bkma_adv_batch *batch, uint64_t nsteps, int32_t u
 bkma_status bkma_adv_batch_run_loop(bkma_adv_batch *batch, uint64_t nsteps, int32_t mode, void *stream);
 bkma_status bkma_adv_batch_describe(const bkma_adv_batch *batch, uint64_t *cells, uint64_t *items,
                                     uint64_t *segmented_items, uint64_t *lds_budget, uint32_t *grid);
+/* Steps per pass of the batch (0 for a null batch). */
+int32_t bkma_adv_batch_steps_per_pass(const bkma_adv_batch *batch);
 void bkma_adv_batch_destroy(bkma_adv_batch *batch);
 
 /* ------------------------------------------------------------------ */

@@ -130,8 +130,9 @@ EXPORTS = [
     "bkma_conv_run", "bkma_fill_uniform", "bkma_fill_normal", "bkma_fill_constant", "bkma_fill_adv_sin4",
     "bkma_validate_adv_l1", "bkma_conv_sum_normalize", "bkma_conv_validate", "bkma_error_norms",
     "bkma_adv_plan_create", "bkma_adv_plan_create_with", "bkma_adv_plan_describe", "bkma_adv_plan_run",
-    "bkma_adv_plan_destroy", "bkma_adv_plan_variants", "bkma_adv_batch_create", "bkma_adv_batch_run",
-    "bkma_adv_batch_run_loop", "bkma_adv_batch_describe", "bkma_adv_batch_destroy",
+    "bkma_adv_plan_destroy", "bkma_adv_plan_variants", "bkma_adv_batch_create", "bkma_adv_batch_create_with",
+    "bkma_adv_batch_run",
+    "bkma_adv_batch_run_loop", "bkma_adv_batch_describe", "bkma_adv_batch_steps_per_pass", "bkma_adv_batch_destroy",
     "bkma_conv1d_plan_create", "bkma_conv1d_plan_run", "bkma_conv1d_plan_describe", "bkma_conv1d_plan_destroy",
     "bkma_conv1d_plan_set_input_scale", "bkma_adv_run_noalloc", "bkma_static_scratch_bytes",
 ]
@@ -180,9 +181,11 @@ def lib():
         "bkma_adv_plan_destroy": (None, [vp]),
         "bkma_adv_plan_variants": (i32, [ctypes.c_int, P(TunedConfig), i32]),
         "bkma_adv_batch_create": (st, [P(AdvProblem), i32, ctypes.c_int, i32, vp, P(vp)]),
+        "bkma_adv_batch_create_with": (st, [P(AdvProblem), i32, ctypes.c_int, u64, i32, i32, vp, P(vp)]),
         "bkma_adv_batch_run": (st, [vp, u64, i32, vp]),
         "bkma_adv_batch_run_loop": (st, [vp, u64, i32, vp]),
         "bkma_adv_batch_describe": (st, [vp, P(u64), P(u64), P(u64), P(u64), P(ctypes.c_uint32)]),
+        "bkma_adv_batch_steps_per_pass": (i32, [vp]),
         "bkma_adv_batch_destroy": (None, [vp]),
         "bkma_conv1d_plan_create": (st, [ctypes.c_int, ctypes.c_int, u64, u64, u64, u64, ctypes.c_int, vp, vp, vp,
                                          P(vp)]),
@@ -387,7 +390,9 @@ class AdvBatch:
     """A ragged batch of advection problems (bkma_adv_batch_*).  `arrays` are
     contiguous device tensors, `params` their AdvParams."""
 
-    def __init__(self, arrays, params, dtype, autotune: bool = False, stream=None):
+    def __init__(self, arrays, params, dtype, autotune: bool = False, stream=None, config=None):
+        """config = (lds_budget, threads, steps_per_pass) builds the batch with that configuration
+        (bkma_adv_batch_create_with) instead of the heuristic or autotuned one."""
         n = len(arrays)
         probs = (AdvProblem * n)()
         for j, (a, p) in enumerate(zip(arrays, params)):
@@ -395,7 +400,13 @@ class AdvBatch:
             probs[j].params = p
         self.arrays = list(arrays)
         self._h = ctypes.c_void_p()
-        check(lib().bkma_adv_batch_create(probs, n, dtype, int(autotune), _stream(stream), ctypes.byref(self._h)))
+        if config is not None:
+            budget, threads, ns = config
+            check(lib().bkma_adv_batch_create_with(probs, n, dtype, budget, threads, ns, _stream(stream),
+                                                   ctypes.byref(self._h)))
+        else:
+            check(lib().bkma_adv_batch_create(probs, n, dtype, int(autotune), _stream(stream),
+                                              ctypes.byref(self._h)))
 
     def run(self, nsteps: int, graph: bool = True, stream=None) -> None:
         check(lib().bkma_adv_batch_run(self._h, nsteps, int(graph), _stream(stream)))
@@ -408,7 +419,8 @@ class AdvBatch:
         g = ctypes.c_uint32()
         check(lib().bkma_adv_batch_describe(self._h, *[ctypes.byref(x) for x in v], ctypes.byref(g)))
         return {"cells": v[0].value, "items": v[1].value, "segmented_items": v[2].value,
-                "lds_budget": v[3].value, "grid": g.value}
+                "lds_budget": v[3].value, "grid": g.value,
+                "steps_per_pass": lib().bkma_adv_batch_steps_per_pass(self._h)}
 
     def close(self) -> None:
         if self._h:

@@ -6,7 +6,10 @@
 // into segments, and an item is a chunk of consecutive segments of one slab
 // that a work-group walks in order (line_chunk_unit): only the halo rows
 // around the chunk come from a snapshot buffer filled by batch_halo_kernel
-// just before.  One step of the batch is
+// just before.  One pass of the batch advances every problem by ns steps
+// (steps fused in LDS, intermediate steps in FP64 as in the LINE_LDS plans:
+// whole lines by line_steps_unit, chunks by line_chunk_steps_unit with halos
+// of ns steps):
 //   batch_halo_kernel  (only if some problem is segmented)
 //   adv_batch_kernel   (persistent: work-groups pull items from a counter)
 // and runs of steps are captured into hipGraphs of up to 32 steps and replayed.
@@ -41,10 +44,13 @@ struct BatchItem {
     i64 i0;
 };
 
-template <class T, int NT>
+// FUSED: passes of ns >= 2 steps (256-thread work-groups); a kernel of its
+// own, so that the one-step kernel keeps its registers and occupancy.
+template <class T, int NT, bool FUSED>
 __global__ __launch_bounds__(NT) void adv_batch_kernel(const BatchProblemDev *__restrict__ probs,
                                                         const BatchItem *__restrict__ items, i64 nitems,
-                                                        unsigned long long *counter) {
+                                                        unsigned long long *counter, int ns) {
+    static_assert(!FUSED || NT == 256, "fused passes run in 256-thread work-groups");
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ i64 s_item;
     const i64 modulus = nitems + (i64)gridDim.x;
@@ -58,16 +64,27 @@ __global__ __launch_bounds__(NT) void adv_batch_kernel(const BatchProblemDev *__
         const BatchProblemDev &pr = probs[item.prob];
         const i64 n1 = pr.p.n1, n2 = pr.p.n2;
         T *slab = reinterpret_cast<T *>(pr.data) + item.i0 * n1 * n2;
+        T *st = reinterpret_cast<T *>(lds_raw);
         if (pr.nseg > 1) {
             const i64 chunk_rows = pr.seg_rows * pr.chunk_segs;
             const i64 c0 = (i64)item.chunk * chunk_rows;
             const i64 c1 = min(n1, c0 + chunk_rows);
             const T *hsnap = reinterpret_cast<const T *>(pr.halo) +
-                             (item.i0 * pr.nchunk + item.chunk) * (pr.hb_max + pr.ha_max) * n2;
-            line_chunk_unit<T>(slab, hsnap, pr.p, item.i0, c0, c1, pr.seg_rows, pr.hb_max, pr.ha_max,
-                               reinterpret_cast<T *>(lds_raw));
+                             (item.i0 * pr.nchunk + item.chunk) * ns * (pr.hb_max + pr.ha_max) * n2;
+            if constexpr (FUSED) {
+                // staged rows, then two FP64 buffers of as many rows (16-byte aligned)
+                const i64 rows = ns * (pr.hb_max + pr.ha_max) + pr.seg_rows;
+                double *w0 = reinterpret_cast<double *>(lds_raw + ((rows * n2 * (i64)sizeof(T) + 15) & ~(i64)15));
+                line_chunk_steps_unit<T>(slab, hsnap, pr.p, item.i0, c0, c1, pr.seg_rows, pr.hb_max, pr.ha_max, ns,
+                                         st, w0, w0 + rows * n2);
+            } else {
+                line_chunk_unit<T>(slab, hsnap, pr.p, item.i0, c0, c1, pr.seg_rows, pr.hb_max, pr.ha_max, st);
+            }
+        } else if constexpr (FUSED) {
+            double *w0 = reinterpret_cast<double *>(lds_raw + ((n1 * n2 * (i64)sizeof(T) + 15) & ~(i64)15));
+            line_steps_unit<T, line_feet_rows<NT>()>(slab, pr.p, item.i0, 1, ns, st, w0, w0 + n1 * n2);
         } else {
-            line_lds_unit<T, false>(slab, nullptr, pr.p, item.i0, 0, n1, 0, 0, reinterpret_cast<T *>(lds_raw));
+            line_lds_unit<T, false>(slab, nullptr, pr.p, item.i0, 0, n1, 0, 0, st);
         }
         __syncthreads();  // LDS is reused by the next item
     }
@@ -76,7 +93,8 @@ __global__ __launch_bounds__(NT) void adv_batch_kernel(const BatchProblemDev *__
 // Halo snapshot of the segmented items (one work-group per item): the hb
 // rows before and the ha rows after the item's chunk.
 template <class T>
-__global__ void batch_halo_kernel(const BatchProblemDev *__restrict__ probs, const BatchItem *__restrict__ items) {
+__global__ void batch_halo_kernel(const BatchProblemDev *__restrict__ probs, const BatchItem *__restrict__ items,
+                                  int ns) {
     const BatchItem item = items[blockIdx.x];
     const BatchProblemDev &pr = probs[item.prob];
     const i64 n1 = pr.p.n1, n2 = pr.p.n2;
@@ -84,8 +102,10 @@ __global__ void batch_halo_kernel(const BatchProblemDev *__restrict__ probs, con
     const i64 r1 = min(n1, r0 + pr.seg_rows * pr.chunk_segs);
     int hb, ha;
     slab_halo(pr.p, item.i0, pr.hb_max, pr.ha_max, hb, ha);
+    hb *= ns;  // ns fused steps reach ns times as far
+    ha *= ns;
     const T *slab = reinterpret_cast<const T *>(pr.data) + item.i0 * n1 * n2;
-    T *dst = reinterpret_cast<T *>(pr.halo) + (item.i0 * pr.nchunk + item.chunk) * (pr.hb_max + pr.ha_max) * n2;
+    T *dst = reinterpret_cast<T *>(pr.halo) + (item.i0 * pr.nchunk + item.chunk) * ns * (pr.hb_max + pr.ha_max) * n2;
     const i64 total = (i64)(hb + ha) * n2;
     for (i64 e = threadIdx.x; e < total; e += blockDim.x) {
         const i64 hr = e / n2, c = e - hr * n2;
@@ -123,6 +143,7 @@ struct AdvBatch {
     std::vector<void *> halos;
     u64 lds_budget = 0;
     int threads = 256;  // per work-group: 256 or 1024 (the autotuner's choice)
+    int ns = 1;         // steps per pass (fused steps need 256-thread work-groups)
     unsigned grid = 0;
     hipStream_t stream = nullptr;  // internal stream (graph capture and replay)
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -144,28 +165,41 @@ struct AdvBatch {
     }
 };
 
-template <class T> static bkma_status batch_step(const AdvBatch &b, hipStream_t s) {
+// One pass of ns <= b.ns steps (the items, halo buffers and LDS budget are
+// sized for b.ns, and hold for fewer steps).
+template <class T> static bkma_status batch_pass(const AdvBatch &b, hipStream_t s, int ns) {
     if (!b.host_seg_items.empty()) {
         hipLaunchKernelGGL(batch_halo_kernel<T>, dim3((unsigned)b.host_seg_items.size()), dim3(256), 0, s,
-                           (const BatchProblemDev *)b.d_probs, (const BatchItem *)b.d_seg_items);
+                           (const BatchProblemDev *)b.d_probs, (const BatchItem *)b.d_seg_items, ns);
         BKMA_TRY(check_launch("batch_halo_kernel"));
     }
-    auto k = b.threads == 1024 ? adv_batch_kernel<T, 1024> : adv_batch_kernel<T, 256>;
+    // a pass of one step (a run's remainder) runs the one-step kernel: the
+    // fused chunk unit needs ns >= 2
+    auto k = ns > 1 ? adv_batch_kernel<T, 256, true>
+                    : (b.threads == 1024 ? adv_batch_kernel<T, 1024, false> : adv_batch_kernel<T, 256, false>);
     hipLaunchKernelGGL(k, dim3(b.grid), dim3(b.threads), (size_t)b.lds_budget, s, (const BatchProblemDev *)b.d_probs,
-                       (const BatchItem *)b.d_items, (i64)b.host_items.size(), b.d_counter);
+                       (const BatchItem *)b.d_items, (i64)b.host_items.size(), b.d_counter, ns);
     return check_launch("adv_batch_kernel");
 }
 
+// nsteps steps: passes of b.ns steps, then one pass of the remainder.
 static bkma_status batch_steps(const AdvBatch &b, u64 nsteps, hipStream_t s) {
-    for (u64 t = 0; t < nsteps; ++t)
-        BKMA_TRY(b.dtype == BKMA_F64 ? batch_step<double>(b, s) : batch_step<float>(b, s));
+    for (u64 t = 0; t < nsteps;) {
+        const int k = (int)std::min<u64>((u64)b.ns, nsteps - t);
+        BKMA_TRY(b.dtype == BKMA_F64 ? batch_pass<double>(b, s, k) : batch_pass<float>(b, s, k));
+        t += (u64)k;
+    }
     return BKMA_OK;
 }
 
-// Build the item lists for an LDS budget (bytes per work-group) and
-// work-groups of `threads` threads.
-static bkma_status build_items(AdvBatch &b, u64 budget, int threads) {
+// Build the item lists for an LDS budget (bytes per work-group), work-groups
+// of `threads` threads and ns steps per pass.  With ns > 1 a staged row takes
+// two FP64 rows besides its own, and a chunk's halos are ns steps wide.
+static bkma_status build_items(AdvBatch &b, u64 budget, int threads, int ns) {
+    if (ns < 1 || (ns > 1 && threads > 256))  // the kernel fuses steps in 256-thread work-groups only
+        return fail(BKMA_ERR_UNSUPPORTED, "advection batch: fused steps need 256-thread work-groups");
     const u64 esz = b.dtype == BKMA_F64 ? 8 : 4;
+    const u64 row_esz = esz + (ns > 1 ? 16 : 0);  // LDS bytes per staged cell
     for (void *h : b.halos) (void)hipFree(h);
     b.halos.clear();
     b.host_probs.clear();
@@ -176,7 +210,7 @@ static bkma_status build_items(AdvBatch &b, u64 budget, int threads) {
         BatchProblemDev d{};
         d.data = b.problems[j].data;
         d.p = make_adv_dev(p);
-        const u64 slab_bytes = p.n1 * p.n2 * esz;
+        const u64 slab_bytes = p.n1 * p.n2 * row_esz + 16;
         if (slab_bytes <= budget) {
             d.seg_rows = (i64)p.n1;
             d.nseg = 1;
@@ -186,7 +220,7 @@ static bkma_status build_items(AdvBatch &b, u64 budget, int threads) {
             shift_bounds(p, &dlo, &dhi);
             d.hb_max = std::max(0, dhi + 2);
             d.ha_max = std::max(0, 3 - dlo);
-            const i64 rows = (i64)(budget / (p.n2 * esz)) - d.hb_max - d.ha_max;
+            const i64 rows = (i64)((budget - 16) / (p.n2 * row_esz)) - (i64)ns * (d.hb_max + d.ha_max);
             if (rows < 16)
                 return fail(BKMA_ERR_UNSUPPORTED, "advection batch: problem " + std::to_string(j) +
                                                       " has rows too wide or a shift too large for the LDS budget");
@@ -194,10 +228,10 @@ static bkma_status build_items(AdvBatch &b, u64 budget, int threads) {
             d.nseg = (i64)ceil_div(p.n1, (u64)d.seg_rows);
             // chain up to kBatchChunkSegs segments per item (needs segments at
             // least as long as the halo before them)
-            d.chunk_segs = d.seg_rows >= d.hb_max ? std::min<i64>(d.nseg, kBatchChunkSegs) : 1;
+            d.chunk_segs = d.seg_rows >= (i64)ns * d.hb_max ? std::min<i64>(d.nseg, kBatchChunkSegs) : 1;
             d.nchunk = (i64)ceil_div((u64)d.nseg, (u64)d.chunk_segs);
             void *h = nullptr;
-            const u64 hb = p.n0 * (u64)d.nchunk * (u64)(d.hb_max + d.ha_max) * p.n2 * esz;
+            const u64 hb = p.n0 * (u64)d.nchunk * (u64)ns * (u64)(d.hb_max + d.ha_max) * p.n2 * esz;
             BKMA_HIP_TRY(hipMalloc(&h, std::max<u64>(hb, 16)));
             b.halos.push_back(h);
             d.halo = h;
@@ -237,14 +271,17 @@ static bkma_status build_items(AdvBatch &b, u64 budget, int threads) {
     // Persistent grid: every work-group that can be resident at this LDS size.
     b.lds_budget = budget;
     b.threads = threads;
+    b.ns = ns;
     int dev = 0, cus = 0, blocks = 0;
     BKMA_HIP_TRY(hipGetDevice(&dev));
     BKMA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const void *k = b.dtype == BKMA_F64
-                        ? (threads == 1024 ? (const void *)adv_batch_kernel<double, 1024>
-                                           : (const void *)adv_batch_kernel<double, 256>)
-                        : (threads == 1024 ? (const void *)adv_batch_kernel<float, 1024>
-                                           : (const void *)adv_batch_kernel<float, 256>);
+                        ? (ns > 1 ? (const void *)adv_batch_kernel<double, 256, true>
+                                  : (threads == 1024 ? (const void *)adv_batch_kernel<double, 1024, false>
+                                                     : (const void *)adv_batch_kernel<double, 256, false>))
+                        : (ns > 1 ? (const void *)adv_batch_kernel<float, 256, true>
+                                  : (threads == 1024 ? (const void *)adv_batch_kernel<float, 1024, false>
+                                                     : (const void *)adv_batch_kernel<float, 256, false>));
     BKMA_HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)budget));
     BKMA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, threads, (size_t)budget));
     b.grid = (unsigned)std::max<i64>(1, std::min<i64>((i64)blocks * cus, (i64)b.host_items.size()));
@@ -261,14 +298,13 @@ using namespace bkma;
 
 struct bkma_adv_batch : AdvBatch {};
 
-extern "C" bkma_status bkma_adv_batch_create(const bkma_adv_problem *problems, int32_t nproblems, bkma_dtype dtype,
-                                             int32_t autotune, void *stream, bkma_adv_batch **out) {
+// Validation, counter, stream and events of a new batch (no items yet).
+static bkma_status batch_setup(const bkma_adv_problem *problems, int32_t nproblems, bkma_dtype dtype,
+                               bkma_adv_batch **out) {
     if (!problems || !out || nproblems <= 0) return fail(BKMA_ERR_INVALID_ARGUMENT, "advection batch: no problems");
     *out = nullptr;
     if (dtype != BKMA_F32 && dtype != BKMA_F64) return fail(BKMA_ERR_UNSUPPORTED, "advection batch: F32 or F64");
     BKMA_TRY(require_device());
-    u64 lds_max = 0;
-    BKMA_TRY(lds_limit_bytes(&lds_max));
     auto *b = new (std::nothrow) bkma_adv_batch();
     if (!b) return fail(BKMA_ERR_OUT_OF_MEMORY, "batch allocation");
     b->dtype = dtype;
@@ -285,7 +321,6 @@ extern "C" bkma_status bkma_adv_batch_create(const bkma_adv_problem *problems, i
         b->problems.push_back(problems[j]);
         b->cells += p.n0 * p.n1 * p.n2;
     }
-    bkma_status st = BKMA_OK;
     auto bail = [&](bkma_status s) {
         delete b;
         return s;
@@ -296,6 +331,42 @@ extern "C" bkma_status bkma_adv_batch_create(const bkma_adv_problem *problems, i
         hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) != hipSuccess)
         return bail(fail(BKMA_ERR_DEVICE, "batch stream/event creation"));
+    *out = b;
+    return BKMA_OK;
+}
+
+extern "C" bkma_status bkma_adv_batch_create_with(const bkma_adv_problem *problems, int32_t nproblems,
+                                                  bkma_dtype dtype, uint64_t lds_budget, int32_t threads,
+                                                  int32_t steps_per_pass, void *stream, bkma_adv_batch **out) {
+    (void)stream;
+    u64 lds_max = 0;
+    BKMA_TRY(lds_limit_bytes(&lds_max));
+    if ((threads != 256 && threads != 1024) || steps_per_pass < 1 || steps_per_pass > 4 || lds_budget < 1024 ||
+        lds_budget > lds_max - 1024)
+        return fail(BKMA_ERR_INVALID_ARGUMENT, "advection batch: threads 256 or 1024, 1 to 4 steps per pass, "
+                                               "an LDS budget from 1 KB to the device limit less 1 KB");
+    bkma_adv_batch *b = nullptr;
+    BKMA_TRY(batch_setup(problems, nproblems, dtype, &b));
+    const bkma_status st = build_items(*b, lds_budget, threads, steps_per_pass);
+    if (st != BKMA_OK) {
+        delete b;
+        return st;
+    }
+    *out = b;
+    return BKMA_OK;
+}
+
+extern "C" bkma_status bkma_adv_batch_create(const bkma_adv_problem *problems, int32_t nproblems, bkma_dtype dtype,
+                                             int32_t autotune, void *stream, bkma_adv_batch **out) {
+    u64 lds_max = 0;
+    BKMA_TRY(lds_limit_bytes(&lds_max));
+    bkma_adv_batch *b = nullptr;
+    BKMA_TRY(batch_setup(problems, nproblems, dtype, &b));
+    bkma_status st = BKMA_OK;
+    auto bail = [&](bkma_status s) {
+        delete b;
+        return s;
+    };
 
     // The timing runs below use the internal stream: order them after the
     // caller's pending work on `stream` (its writes to the problems' arrays).
@@ -305,47 +376,57 @@ extern "C" bkma_status bkma_adv_batch_create(const bkma_adv_problem *problems, i
             return bail(fail(BKMA_ERR_DEVICE, "batch: ordering after the caller's stream"));
     }
 
-    // Dynamic LDS budgets per work-group (the kernel adds a few static bytes).
+    // Dynamic LDS budgets per work-group (the kernel adds a few static bytes),
+    // work-group sizes and steps per pass.  Without autotuning: 64 KB, 256
+    // threads, one step per pass.
     const u64 cap = lds_max - 1024;
     std::vector<u64> budgets = {std::min<u64>(65536, cap)};
-    if (autotune) budgets = {32768, 65536, 98304, cap};
+    std::vector<int> passes = {1};
+    if (autotune) {
+        budgets = {32768, 65536, 98304, cap};
+        passes = {1, 2, 3, 4};
+    }
     u64 best_budget = 0;
-    int best_threads = 256;
+    int best_threads = 256, best_ns = 1;
     float best_ms = 1e30f;
     bool done = false;
     for (u64 bud : budgets) {
         for (int thr : {256, 1024}) {
-            if (done || bud > lds_max) break;
-            if (build_items(*b, bud, thr) != BKMA_OK) continue;
-            if (!autotune) {
-                best_budget = bud;
-                done = true;
-                break;
+            for (int ns : passes) {
+                if (done || bud > lds_max || (ns > 1 && thr > 256)) break;
+                if (build_items(*b, bud, thr, ns) != BKMA_OK) continue;
+                if (!autotune) {
+                    best_budget = bud;
+                    done = true;
+                    break;
+                }
+                // The timing runs update the problems in place: the autotuner is
+                // meant to run before the data is initialised (or on scratch data).
+                hipEvent_t e0, e1;
+                (void)hipEventCreate(&e0);
+                (void)hipEventCreate(&e1);
+                st = batch_steps(*b, (u64)ns, b->stream);
+                (void)hipEventRecord(e0, b->stream);
+                if (st == BKMA_OK) st = batch_steps(*b, 3 * (u64)ns, b->stream);
+                (void)hipEventRecord(e1, b->stream);
+                (void)hipEventSynchronize(e1);
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                ms /= (float)ns;  // per 3 steps
+                if (st == BKMA_OK && ms < best_ms) {
+                    best_ms = ms;
+                    best_budget = bud;
+                    best_threads = thr;
+                    best_ns = ns;
+                }
+                st = BKMA_OK;
             }
-            // The timing runs update the problems in place: the autotuner is
-            // meant to run before the data is initialised (or on scratch data).
-            hipEvent_t e0, e1;
-            (void)hipEventCreate(&e0);
-            (void)hipEventCreate(&e1);
-            st = batch_steps(*b, 1, b->stream);
-            (void)hipEventRecord(e0, b->stream);
-            if (st == BKMA_OK) st = batch_steps(*b, 3, b->stream);
-            (void)hipEventRecord(e1, b->stream);
-            (void)hipEventSynchronize(e1);
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            if (st == BKMA_OK && ms < best_ms) {
-                best_ms = ms;
-                best_budget = bud;
-                best_threads = thr;
-            }
-            st = BKMA_OK;
         }
     }
     if (best_budget == 0) return bail(fail(BKMA_ERR_UNSUPPORTED, "advection batch: no LDS budget covers the problems"));
-    st = build_items(*b, best_budget, best_threads);
+    st = build_items(*b, best_budget, best_threads, best_ns);
     if (st != BKMA_OK) return bail(st);
     if (autotune) {  // later work on the caller's stream sees the timing runs' updates
         if (hipEventRecord(b->ev_out, b->stream) != hipSuccess ||
@@ -454,6 +535,8 @@ extern "C" bkma_status bkma_adv_batch_describe(const bkma_adv_batch *b, uint64_t
     if (grid) *grid = b->grid;
     return BKMA_OK;
 }
+
+extern "C" int32_t bkma_adv_batch_steps_per_pass(const bkma_adv_batch *b) { return b ? b->ns : 0; }
 
 extern "C" void bkma_adv_batch_destroy(bkma_adv_batch *b) {
     if (b) (void)hipStreamSynchronize(b->stream);

@@ -43,12 +43,17 @@ def _fill(arrays, seed0):
     return [a.cpu().numpy() for a in arrays]
 
 
-def _oracle(hosts, specs, dtype, nsteps):
+def _oracle(hosts, specs, dtype, nsteps, ns=1):
+    """One batch run of nsteps steps: passes of ns steps, then the rest (FP32: one rounding per pass; FP64
+    storage keeps every intermediate step in FP64 anyway)."""
     refs = []
     for h, (n0, n1, n2, dt) in zip(hosts, specs):
         r = h.copy()
         po = O.make_params(n0, n1, n2, dt=dt, minRealVx=-1, maxRealVx=1)
-        (O.advect_f64 if dtype == B.F64 else O.advect_f32_mixed)(r, po, nsteps)
+        if dtype == B.F64:
+            O.advect_f64(r, po, nsteps)
+        else:
+            O.advect_f32_blocked(r, po, nsteps, ns)
         refs.append(r)
     return refs
 
@@ -61,17 +66,49 @@ def test_batch_graph_replays_bit_exact(cuda, dtype, autotune):
     print("specs", specs, flush=True)
     arrays, params = _make(dtype, specs, 100)
     batch = B.AdvBatch(arrays, params, dtype, autotune=autotune)  # before the data is set
-    print(batch.describe(), flush=True)
-    hosts = _fill(arrays, 100)
+    info = batch.describe()
+    print(info, flush=True)
+    refs = _fill(arrays, 100)
     for rep in range(3):
         batch.run(4, graph=True)
         torch.cuda.synchronize()
         # eager work between replays (a plain copy and a norm)
         _ = arrays[0].clone().sum().item()
-        refs = _oracle(hosts, specs, dtype, 4 * (rep + 1))
+        refs = _oracle(refs, specs, dtype, 4, info["steps_per_pass"])
         for j, (a, r) in enumerate(zip(arrays, refs)):
             assert np.array_equal(a.cpu().numpy(), r), (rep, j, specs[j])
     batch.close()
+
+
+# Whole-line items (one with its rows' feet kept across the steps, one without) and chained segments, advanced
+# by fused passes; runs of 5 steps (graph) and 3 steps (no graph) end with a shorter pass.
+FUSED_SPECS = [(3, 8192, 8, 2e-3), (2, 16384, 1, 1e-3), (4, 640, 64, 1e-3), (5, 100, 1, 2e-3),
+               (3, 2048, 1, 1.5e-3)]
+
+
+@pytest.mark.parametrize("dtype", [B.F32, B.F64], ids=["f32", "f64"])
+@pytest.mark.parametrize("ns", [2, 3, 4])
+def test_batch_fused_passes_bit_exact(cuda, dtype, ns):
+    import torch
+    arrays, params = _make(dtype, FUSED_SPECS, 400)
+    batch = B.AdvBatch(arrays, params, dtype, config=(98304, 256, ns))
+    info = batch.describe()
+    print(info, flush=True)
+    assert info["steps_per_pass"] == ns and info["segmented_items"] > 0 and info["items"] > info["segmented_items"]
+    refs = _fill(arrays, 400)
+    for nsteps, graph in ((5, True), (3, False), (5, True)):
+        batch.run(nsteps, graph=graph)
+        torch.cuda.synchronize()
+        refs = _oracle(refs, FUSED_SPECS, dtype, nsteps, ns)
+        for j, (a, r) in enumerate(zip(arrays, refs)):
+            assert np.array_equal(a.cpu().numpy(), r), (nsteps, j, FUSED_SPECS[j])
+    batch.close()
+
+
+def test_batch_fused_passes_need_256_threads(cuda):
+    arrays, params = _make(B.F32, FUSED_SPECS[:2], 500)
+    with pytest.raises(B.BkmaError):
+        B.AdvBatch(arrays, params, B.F32, config=(98304, 1024, 2))
 
 
 def test_batch_without_graph_and_segments(cuda):
