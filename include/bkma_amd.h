@@ -207,16 +207,21 @@ typedef enum bkma_strategy {
                                         (checked per slab at plan creation): seven-row windows, no shift branches */
     BKMA_STRATEGY_PLAIN = 4,         /* the reference's AdaptiveWg executor and create_optim_params dispatch
                                         (F64 storage, line in LDS; seg_rows = pref_wg_size), one step per pass */
-    BKMA_STRATEGY_HYBRID = 5         /* the reference's local/global split (HybridMem.cpp:4-32, percent_loc of
+    BKMA_STRATEGY_HYBRID = 5,        /* the reference's local/global split (HybridMem.cpp:4-32, percent_loc of
                                         AdvectionParams.hpp:18): the first floor(n0 * chunk_segs / 100) slabs run
                                         the AdaptiveWg executor with the line scratch in LDS, the other slabs with
                                         a global scratch, concurrently on a second stream (F64 storage;
                                         seg_rows = pref_wg_size), one step per pass */
+    BKMA_STRATEGY_LINE_SHFL = 6      /* contiguous lines (n2 = 1) of 256 .. 4096 cells (a power of two) whose rows
+                                        take at most two adjacent shifts: one work-group holds a line in registers
+                                        (vec = 4 or 8 cells per lane, 0 = 8; waves_per_wg = n1 / (64 vec), at most
+                                        8 with 8 cells) for up to 256 steps per pass, stencil neighbours through DPP
+                                        wavefront shifts; shifts up to about vec - 3 cells */
 } bkma_strategy;
 
 typedef struct bkma_tuned_config {
     int32_t strategy;     /* bkma_strategy */
-    int32_t vec;          /* consecutive i2 cells per lane */
+    int32_t vec;          /* consecutive i2 cells per lane (LINE_SHFL: consecutive i1 cells per lane) */
     int32_t smax;         /* largest |v*dt/dx| the stencil window covers */
     int32_t prefetch;     /* rows loaded ahead of the stencil window */
     int32_t waves_per_wg; /* wavefronts per work-group (LINE_LDS: 4, 8 or 16; 0 = 4) */
@@ -224,8 +229,8 @@ typedef struct bkma_tuned_config {
     int32_t persistent;   /* 1: grid capped at the resident work-groups, waves loop over units */
     int32_t seg_rows;     /* LINE_LDS: rows per segment (0 = whole line in LDS); PLAIN, HYBRID: pref_wg_size */
     int32_t wide_window;  /* COLUMN_STREAM, F32 storage: 1 = FP64 register window (rows widened once) */
-    int32_t steps_per_pass; /* COLUMN_STREAM, COLUMN_PAIR, LINE_LDS whole line (at most 4): time steps fused into
-                               one pass over HBM (0 or 1: one) */
+    int32_t steps_per_pass; /* COLUMN_STREAM, COLUMN_PAIR, LINE_LDS (at most 4), LINE_SHFL (at most 256): time
+                               steps fused into one pass over HBM (0 or 1: one) */
     int32_t chunk_segs;   /* LINE_LDS segmented: consecutive segments one work-group walks in order (0 or 1: one);
                              LINE_LDS whole line: consecutive slabs one work-group stages together;
                              HYBRID: percent of the slabs whose line scratch is in LDS (0..100) */

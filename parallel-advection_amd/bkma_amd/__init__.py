@@ -108,6 +108,7 @@ STRATEGY_LINE_LDS = 2
 STRATEGY_COLUMN_PAIR = 3
 STRATEGY_PLAIN = 4
 STRATEGY_HYBRID = 5
+STRATEGY_LINE_SHFL = 6
 
 
 class AdvProblem(ctypes.Structure):

@@ -7,7 +7,10 @@
 // Strategy COLUMN_PAIR: the column stream for slabs whose rows take at most
 // two adjacent shifts (adv_pair.hpp): seven-tap windows, no shift branches.
 // Strategy LINE_LDS (any n2, any shift): slabs or segments of them staged in
-// LDS (adv_lds.hpp), one step per pass.
+// LDS (adv_lds.hpp), up to 4 steps per pass.
+// Strategy LINE_SHFL (contiguous lines, n2 = 1, of 256 .. 4096 cells): each
+// line held in registers by one work-group for a pass of up to 256 steps,
+// stencil neighbours through DPP wavefront shifts (adv_shfl.hpp).
 //
 // FP64 storage: results bit-identical to the reference's FP64 steps.
 // FP32 storage: FP64 foot location / index / lag_basis exactly as the
@@ -24,10 +27,13 @@
 
 #include "adv_lds.hpp"
 #include "adv_pair.hpp"
+#include "adv_shfl.hpp"
 #include "adv_stream.hpp"
 #include "common.hpp"
 
 namespace bkma {
+
+constexpr int kShiftStats = 5;  // entries of the slab shift statistics
 
 __global__ void slab_shift_scan_kernel(AdvDev p, int *__restrict__ shift, int *__restrict__ stats) {
     __shared__ int smin, smax;
@@ -57,6 +63,7 @@ __global__ void slab_shift_scan_kernel(AdvDev p, int *__restrict__ shift, int *_
             atomicMin(&stats[0], smin);
             atomicMax(&stats[1], smax);
             atomicMax(&stats[2], smax - smin);
+            atomicMin(&stats[smax == smin ? 3 : 4], smax);
         }
         __syncthreads();
     }
@@ -112,7 +119,8 @@ struct TunedPlan {
     const StreamVariant *variant = nullptr;
     int ns = 1;                        // steps per pass
     i64 persistent_blocks[9] = {};    // by steps per pass: 0 = one wave per unit, else the grid cap
-    int *slab_shift = nullptr;         // COLUMN_PAIR: smallest and largest shift of each slab, [n0][2] (owned)
+    int *slab_shift = nullptr;         // COLUMN_PAIR, LINE_SHFL: smallest and largest shift of each slab, [n0][2] (owned)
+    int shfl_waves = 0;                // LINE_SHFL: wavefronts per line
     bkma_optim_params optim{};         // PLAIN, HYBRID: the reference's dispatch for its AdaptiveWg executor
     // HYBRID: slabs [0, k_local) with their lines' scratch in LDS (optim),
     // [k_local, n0) with a global scratch (optim_global, gscratch), the two
@@ -169,12 +177,14 @@ bool shift_supported(const bkma_adv_params &p, std::string *why) {
 }
 
 // Shifts of every (i0, i1) from the device's foot arithmetic: stats =
-// {smallest shift, largest shift, largest spread within one slab}; the
+// {smallest shift, largest shift, largest spread within one slab, smallest
+// shift of the slabs with one shift, smallest largest shift of the slabs with
+// several (INT_MAX where there are none)}; the
 // per-slab smallest and largest shifts ([n0][2]) go to *dlo_out (device
 // memory, owned by the caller) when it is not null.
-bkma_status scan_slab_shifts(const bkma_adv_params &p, int **dlo_out, int stats[3]) {
+bkma_status scan_slab_shifts(const bkma_adv_params &p, int **dlo_out, int stats[kShiftStats]) {
     int *dlo = nullptr, *dst = nullptr;
-    const int init[3] = {INT_MAX, INT_MIN, 0};
+    const int init[kShiftStats] = {INT_MAX, INT_MIN, 0, INT_MAX, INT_MAX};
     if (hipMalloc(&dlo, std::max<u64>(p.n0, 1) * 2 * sizeof(int)) != hipSuccess) {
         (void)hipGetLastError();
         return fail(BKMA_ERR_OUT_OF_MEMORY, "tuned advection: slab shift table allocation failed");
@@ -204,7 +214,7 @@ bkma_status scan_slab_shifts(const bkma_adv_params &p, int **dlo_out, int stats[
 
 // Column-pair feasibility given the slab shift statistics.
 bkma_status pair_feasible(const bkma_adv_params &p, bkma_dtype dt, const StreamVariant &v, int ns,
-                          const int stats[3], std::string *why) {
+                          const int stats[kShiftStats], std::string *why) {
     if (v.dtype != dt || !v.pair) return BKMA_ERR_UNSUPPORTED;
     if (ns < 1 || ns > v.nsmax) {
         if (why) *why = "steps per pass not compiled for this variant";
@@ -253,6 +263,49 @@ bkma_status stream_feasible(const bkma_adv_params &p, bkma_dtype dt, const Strea
         if (why) *why = "line shorter than the register ring";
         return BKMA_ERR_UNSUPPORTED;
     }
+    return BKMA_OK;
+}
+
+// LINE_SHFL feasibility for `cells` consecutive cells per lane: contiguous
+// lines of 64 * cells * NW cells (NW = 1, 2, 4 or 8 wavefronts; 16 with 4
+// cells), every line's rows within two adjacent shifts, and every line's
+// window start -dmax - 2 in the range the lanes g-1 .. g+1 cover (stats: the
+// slab shift statistics; lines with two shifts need one cell more).
+bkma_status shfl_feasible(const bkma_adv_params &p, int cells, int ns, const int stats[kShiftStats], int *nw,
+                          std::string *why) {
+    if (!shift_supported(p, why)) return BKMA_ERR_UNSUPPORTED;
+    if (p.n2 != 1) {
+        if (why) *why = "LINE_SHFL needs contiguous lines (n2 = 1)";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    if (cells != 4 && cells != 8) {
+        if (why) *why = "LINE_SHFL holds 4 or 8 cells per lane";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    const u64 per_wave = 64 * (u64)cells;
+    const u64 w = p.n1 / per_wave;
+    if (p.n1 % per_wave != 0 || (w != 1 && w != 2 && w != 4 && w != 8 && !(w == 16 && cells == 4))) {
+        if (why) *why = cells == 4 ? "LINE_SHFL with 4 cells per lane needs n1 = 256 .. 4096, a power of two"
+                                   : "LINE_SHFL with 8 cells per lane needs n1 = 512 .. 4096, a power of two";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    if (ns < 1 || ns > kShflMaxSteps) {
+        if (why) *why = "LINE_SHFL fuses 1 to 256 steps per pass";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    if (stats[2] > 1) {
+        if (why) *why = "a line takes more than two shifts";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    // window start -dmax - 2 of every line: at least -cells, at most
+    // cells - 5 (six-cell windows, one shift) or cells - 6 (seven, two)
+    const bool low_single = stats[3] != INT_MAX && -stats[3] - 2 > cells - 5;
+    const bool low_pair = stats[4] != INT_MAX && -stats[4] - 2 > cells - 6;
+    if (-stats[1] - 2 < -cells || low_single || low_pair) {
+        if (why) *why = "shift beyond the neighbouring lanes";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    *nw = (int)w;
     return BKMA_OK;
 }
 
@@ -366,6 +419,13 @@ bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s, int ns) 
     }
     if (pl.cfg.strategy == BKMA_STRATEGY_LINE_LDS)
         return pl.dtype == BKMA_F64 ? launch_lds<double>(pl, data, s, ns) : launch_lds<float>(pl, data, s, ns);
+    if (pl.cfg.strategy == BKMA_STRATEGY_LINE_SHFL) {
+        ShflArgs a;
+        a.p = make_adv_dev(pl.params);
+        a.slab_shift = pl.slab_shift;
+        a.ns = ns;
+        return launch_line_shfl(pl.dtype, pl.cfg.vec, pl.shfl_waves, data, a, (i64)pl.params.n0, s);
+    }
     if (ns < 1 || ns > pl.variant->nsmax) return fail(BKMA_ERR_RUNTIME, "tuned advection: steps per pass out of range");
     const StreamVariant &v = *pl.variant;
     StreamArgs a;
@@ -433,7 +493,7 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
                 v.wpb == cfg.waves_per_wg && v.nt == (cfg.nontemporal != 0) && v.wide == (cfg.wide_window != 0) &&
                 v.dpp == (cfg.dpp_broadcast != 0) && ns <= v.nsmax) {
                 if (pair) {
-                    int stats[3];
+                    int stats[kShiftStats];
                     if (!shift_supported(p, &why))
                         return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
                     BKMA_TRY(scan_slab_shifts(p, &pl->slab_shift, stats));
@@ -537,6 +597,23 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
         pl->name = lds_name(dt, pl->lds, pl->lds_threads);
         return BKMA_OK;
     }
+    if (cfg.strategy == BKMA_STRATEGY_LINE_SHFL) {
+        const int ns = steps_per_pass(cfg);
+        int stats[kShiftStats];
+        if (!shift_supported(p, &why))
+            return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
+        const int cells = cfg.vec > 0 ? cfg.vec : 8;
+        BKMA_TRY(scan_slab_shifts(p, &pl->slab_shift, stats));
+        if (shfl_feasible(p, cells, ns, stats, &pl->shfl_waves, &why) != BKMA_OK)
+            return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
+        pl->ns = ns;
+        pl->cfg.steps_per_pass = ns;
+        pl->cfg.vec = cells;
+        pl->cfg.waves_per_wg = pl->shfl_waves;
+        pl->name = std::string("line_shfl<") + (dt == BKMA_F64 ? "f64" : "f32") + ",cells" + std::to_string(cells) +
+                   ",wg" + std::to_string(64 * pl->shfl_waves) + ",steps" + std::to_string(ns) + ">";
+        return BKMA_OK;
+    }
     return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: unknown strategy");
 }
 
@@ -545,7 +622,7 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
 std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype dt, std::string *why) {
     std::vector<bkma_tuned_config> out;
     // slab shift statistics for the column-pair variants (one device scan)
-    int stats[3] = {0, 0, 2};
+    int stats[kShiftStats] = {0, 0, 2, 0, 0};
     bool have_stats = false;
     for (const auto &v : stream_variants(dt))
         if (v.pair && !have_stats && shift_supported(p, nullptr)) {
@@ -578,6 +655,20 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
                 c.persistent = pers;
                 out.push_back(c);
             }
+        }
+    }
+    // contiguous lines in registers, neighbours through DPP
+    for (int cells : {8, 4}) {
+        int nw = 0;
+        if (shfl_feasible(p, cells, 1, stats, &nw, why) != BKMA_OK) continue;
+        for (int ns : {1, 4, 16, 64, 256}) {
+            bkma_tuned_config c;
+            std::memset(&c, 0, sizeof(c));
+            c.strategy = BKMA_STRATEGY_LINE_SHFL;
+            c.vec = cells;
+            c.waves_per_wg = nw;
+            c.steps_per_pass = ns;
+            out.push_back(c);
         }
     }
     const u64 esz = dt == BKMA_F64 ? 8 : 4;
@@ -666,6 +757,8 @@ int heuristic_score(const bkma_tuned_config &c, bkma_dtype dt, const bkma_adv_pa
     if (c.strategy == BKMA_STRATEGY_PLAIN)  // short lines: one work-group per slab would idle
         return (p.n1 < 64 ? 900000000 : 10000000) - std::abs(c.seg_rows - 256);
     if (c.strategy == BKMA_STRATEGY_HYBRID) return 5000000 + c.chunk_segs;
+    if (c.strategy == BKMA_STRATEGY_LINE_SHFL)  // 8 cells per lane, 64 steps per pass
+        return 600000000 + c.vec * 1000 - std::abs(steps_per_pass(c) - 64);
     if (c.strategy == BKMA_STRATEGY_COLUMN_PAIR) {
         // FP32: DPP broadcast, FP64 stage-0 window, VEC 2, P 14, 5 steps;
         // FP64: VEC 2, P 14, 4 steps

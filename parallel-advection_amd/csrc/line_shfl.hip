@@ -1,0 +1,49 @@
+// Instances and launcher of the register/DPP kernel for contiguous lines
+// (adv_shfl.hpp).
+#include "adv_shfl.hpp"
+
+namespace bkma {
+
+namespace {
+
+template <class T, int E, int NW>
+bkma_status launch_nw(void *data, const ShflArgs &a, i64 n0, hipStream_t s) {
+    hipLaunchKernelGGL((adv_line_shfl_kernel<T, E, NW>), dim3((unsigned)n0), dim3(64 * NW), 0, s, (T *)data, a);
+    return check_launch("adv_line_shfl_kernel");
+}
+
+template <class T, int E>
+bkma_status launch_e(int nw, void *data, const ShflArgs &a, i64 n0, hipStream_t s) {
+    switch (nw) {
+        case 1: return launch_nw<T, E, 1>(data, a, n0, s);
+        case 2: return launch_nw<T, E, 2>(data, a, n0, s);
+        case 4: return launch_nw<T, E, 4>(data, a, n0, s);
+        case 8: return launch_nw<T, E, 8>(data, a, n0, s);
+        case 16:
+            if constexpr (E == 4) return launch_nw<T, E, 16>(data, a, n0, s);
+            [[fallthrough]];
+        default: return fail(BKMA_ERR_UNSUPPORTED, "LINE_SHFL: wavefronts per line not compiled");
+    }
+}
+
+template <class T>
+bkma_status launch_t(int cells, int nw, void *data, const ShflArgs &a, i64 n0, hipStream_t s) {
+    if (cells == 4) return launch_e<T, 4>(nw, data, a, n0, s);
+    if (cells == 8) return launch_e<T, 8>(nw, data, a, n0, s);
+    return fail(BKMA_ERR_UNSUPPORTED, "LINE_SHFL: 4 or 8 cells per lane");
+}
+
+}  // namespace
+
+bkma_status launch_line_shfl(bkma_dtype dt, int cells, int nw, void *data, const ShflArgs &a, i64 n0,
+                             hipStream_t s) {
+    if (n0 <= 0 || n0 > 0x7fffffffLL) return fail(BKMA_ERR_UNSUPPORTED, "LINE_SHFL: line count out of range");
+    if (a.ns < 1 || a.ns > kShflMaxSteps) return fail(BKMA_ERR_RUNTIME, "LINE_SHFL: steps per pass out of range");
+    if (a.p.n2 != 1 || a.p.n1 != (i64)64 * cells * nw)
+        return fail(BKMA_ERR_UNSUPPORTED, "LINE_SHFL: line shape does not match the work-group");
+    if (dt == BKMA_F64) return launch_t<double>(cells, nw, data, a, n0, s);
+    if (dt == BKMA_F32) return launch_t<float>(cells, nw, data, a, n0, s);
+    return fail(BKMA_ERR_UNSUPPORTED, "LINE_SHFL: F32 or F64 storage");
+}
+
+}  // namespace bkma

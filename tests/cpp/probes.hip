@@ -53,6 +53,15 @@ __global__ void dpp_wave_shl_kernel(const unsigned *a, unsigned *out) {
     out[l] = (unsigned)__builtin_amdgcn_update_dpp(0, (int)a[l], 0x130, 0xf, 0xf, false);
 }
 
+// DPP whole-wave shift or rotate by one lane (dpp_ctrl CTRL) of a; lanes with
+// no source lane keep old[l] (bound_ctrl off), as the advection line kernel
+// relies on for its neighbouring wave's edge cells.
+template <int CTRL>
+__global__ void dpp_wave_move_kernel(const unsigned *a, const unsigned *old, unsigned *out) {
+    const int l = threadIdx.x;
+    out[l] = (unsigned)__builtin_amdgcn_update_dpp((int)old[l], (int)a[l], CTRL, 0xf, 0xf, false);
+}
+
 int done(const char *) { return hipGetLastError() == hipSuccess ? 0 : 1; }
 
 }  // namespace
@@ -75,6 +84,22 @@ int probe_dpp_wave_shl(const void *a, void *out, void *stream) {
     hipLaunchKernelGGL(dpp_wave_shl_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, (const unsigned *)a,
                        (unsigned *)out);
     return done("dpp_wave_shl_kernel");
+}
+// a[64], old[64] u32 -> out[64]: which = 0 wave_shl:1, 1 wave_rol:1,
+// 2 wave_shr:1, 3 wave_ror:1
+int probe_dpp_wave_move(const void *a, const void *old, void *out, unsigned long long which, void *stream) {
+    auto *pa = (const unsigned *)a;
+    auto *po = (const unsigned *)old;
+    auto *pr = (unsigned *)out;
+    auto st = (hipStream_t)stream;
+    switch (which) {
+        case 0: hipLaunchKernelGGL(dpp_wave_move_kernel<0x130>, dim3(1), dim3(64), 0, st, pa, po, pr); break;
+        case 1: hipLaunchKernelGGL(dpp_wave_move_kernel<0x134>, dim3(1), dim3(64), 0, st, pa, po, pr); break;
+        case 2: hipLaunchKernelGGL(dpp_wave_move_kernel<0x138>, dim3(1), dim3(64), 0, st, pa, po, pr); break;
+        case 3: hipLaunchKernelGGL(dpp_wave_move_kernel<0x13C>, dim3(1), dim3(64), 0, st, pa, po, pr); break;
+        default: return 1;
+    }
+    return done("dpp_wave_move_kernel");
 }
 // in[n] f32 (n even) -> out[n] e4m3 bytes
 int probe_fp8_convert(const void *in, void *out, unsigned long long n, void *stream) {

@@ -45,7 +45,7 @@ def test_binding_covers_header():
 def test_binding_strategy_codes_match_header():
     text = open(os.path.join(ROOT, "include", "bkma_amd.h")).read()
     codes = {m.group(1): int(m.group(2)) for m in re.finditer(r"\bBKMA_(STRATEGY_\w+)\s*=\s*(\d+)", text)}
-    assert len(codes) == 5
+    assert len(codes) == 6
     for name, value in codes.items():
         assert getattr(B, name) == value, name
 
