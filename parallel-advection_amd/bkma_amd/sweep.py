@@ -1,6 +1,7 @@
 """Sweep harness over the reference's benchmark shapes, on one GPU.
 
     python -m bkma_amd.sweep [--sizes e0,e5] [--steps 3] [--skip-conv] [--fp32]
+                             [--tuned-only] [--no-autotune]
 
 Advection (benchmark/advection-bench.cpp:18-96, shapes e0..e9 of
 benchmark/bench_config.hpp:31-42): FP64 data filled by fill_buffer_adv,
@@ -72,16 +73,21 @@ def _timed(torch, fn, steps: int, warmup: int) -> float:
     return (time.perf_counter() - t0) / steps
 
 
-def advection_runs(torch, name, shape, steps, warmup, fp32):
+def _progress(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def advection_runs(torch, name, shape, steps, warmup, fp32, tuned_only=False, autotune=True):
     n0, n1, n2 = shape
     cells = n0 * n1 * n2
     runs = []
-    impls = [("ndrange", B.ND_RANGE, 128), ("ndrange", B.ND_RANGE, 1024),
-             ("adaptivewg", B.ADAPTIVE_WG, 128), ("adaptivewg", B.ADAPTIVE_WG, 1024)]
+    impls = [] if tuned_only else [("ndrange", B.ND_RANGE, 128), ("ndrange", B.ND_RANGE, 1024),
+                                   ("adaptivewg", B.ADAPTIVE_WG, 128), ("adaptivewg", B.ADAPTIVE_WG, 1024)]
     data = torch.empty((n0, n1, n2), dtype=torch.float64, device="cuda")
     for impl_name, impl, w in impls:
         rec = {"kind": "advection", "size": name, "n0": n0, "n1": n1, "n2": n2, "impl": impl_name,
                "pref_wg_size": w, "dtype": "f64"}
+        _progress(f"{name} {impl_name} pref {w}")
         p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1, pref_wg_size=w)
         try:
             o = B.create_optim_params(n0, n1, n2, pref_wg_size=w, elem_size=8)
@@ -100,11 +106,12 @@ def advection_runs(torch, name, shape, steps, warmup, fp32):
         if dt_code == B.F32 and not fp32:
             continue
         rec = {"kind": "advection", "size": name, "n0": n0, "n1": n1, "n2": n2, "impl": "tuned", "dtype": label}
+        _progress(f"{name} tuned {label}" + ("" if autotune else " (heuristic plan)"))
         p = B.make_params(n0, n1, n2, dt=1e-3, minRealVx=-1, maxRealVx=1)
         try:
             d = data if dt_code == B.F64 else torch.empty((n0, n1, n2), dtype=tdt, device="cuda")
             B.fill_adv_sin4(d, p)
-            plan = B.AdvPlan(p, dt_code, autotune=True)
+            plan = B.AdvPlan(p, dt_code, autotune=autotune)
             _, rec["kernel"] = plan.describe()
             spp = max(1, plan.describe()[0].steps_per_pass)
             k = max(steps, spp)  # whole passes
@@ -218,6 +225,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--fp32", action="store_true", help="also time the FP32-storage tuned plan")
     ap.add_argument("--skip-conv", action="store_true")
+    ap.add_argument("--tuned-only", action="store_true", help="time only the tuned plans (no reference executors)")
+    ap.add_argument("--no-autotune", action="store_true", help="tuned plans from the heuristic choice")
     args = ap.parse_args(argv)
     import torch
 
@@ -229,7 +238,8 @@ def main(argv=None) -> int:
         if name not in EXP_SIZES:
             print(f"unknown size {name}", file=sys.stderr)
             return 2
-        adv.append(advection_runs(torch, name, EXP_SIZES[name], args.steps, args.warmup, args.fp32))
+        adv.append(advection_runs(torch, name, EXP_SIZES[name], args.steps, args.warmup, args.fp32,
+                                  tuned_only=args.tuned_only, autotune=not args.no_autotune))
     conv = [] if args.skip_conv else [conv_runs(torch, args.steps, args.warmup)]
     summary = {"kind": "summary", "hbm_peak_GBps": HBM_PEAK_GBS,
                "advection_pp": perf_portability(adv) if adv else {},

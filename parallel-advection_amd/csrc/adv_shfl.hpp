@@ -56,65 +56,82 @@ constexpr int kDppWaveRol1 = 0x134;  // lane l reads lane (l+1) mod 64
 constexpr int kDppWaveShr1 = 0x138;  // lane l reads lane l-1
 constexpr int kDppWaveRor1 = 0x13C;  // lane l reads lane (l-1) mod 64
 
-// ns steps of one line held in f (this lane's E cells), windows of TAPS
-// cells starting at offset DMIN.  xch: 2 * NW * 2 * E doubles of LDS (two
-// step parities x NW waves x {lane 0's cells, lane 63's cells}).
+// One step: g from f (this lane's E cells), windows of TAPS cells starting
+// at offset DMIN; buf is this step's half of the LDS exchange record (NW
+// waves x {lane 0's cells, lane 63's cells}).
 template <class T, int E, int NW, int TAPS, int DMIN>
-__device__ __forceinline__ void shfl_steps(double (&f)[E], const double (&c)[E][TAPS], int ns, double *xch, int w,
-                                           int lane) {
+__device__ __forceinline__ void shfl_step(const double (&f)[E], double (&g)[E], const double (&c)[E][TAPS],
+                                          double *buf, int w, int wl, int wr, int lane) {
     constexpr int NL = DMIN < 0 ? -DMIN : 0;                     // cells from lane g-1: its slots E-NL .. E-1
     constexpr int NR = DMIN + TAPS - 1 > 0 ? DMIN + TAPS - 1 : 0;  // cells from lane g+1: its slots 0 .. NR-1
     static_assert(DMIN >= shfl_dmin_lo<E, TAPS>() && DMIN <= shfl_dmin_hi<E, TAPS>(),
                   "window outside lanes g-1 .. g+1");
+    double hl[NL > 0 ? NL : 1], hr[NR > 0 ? NR : 1];
+    if constexpr (NW > 1) {
+        // edge cells of the neighbouring waves (periodic over the line)
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < NR; ++j) buf[(2 * w) * E + j] = f[j];
+        }
+        if (lane == 63) {
+#pragma unroll
+            for (int j = 0; j < NL; ++j) buf[(2 * w + 1) * E + E - NL + j] = f[E - NL + j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < NL; ++j) hl[j] = buf[(2 * wl + 1) * E + E - NL + j];
+#pragma unroll
+        for (int j = 0; j < NR; ++j) hr[j] = buf[(2 * wr) * E + j];
+    }
+    // W[t] = the cell at offset DMIN + t from this lane's first cell
+    double W[E + TAPS - 1];
+    static_for<0, E + TAPS - 1>([&](auto tt) {
+        constexpr int t = decltype(tt)::value;
+        constexpr int o = DMIN + t;
+        if constexpr (o < 0) {
+            if constexpr (NW > 1) W[t] = dpp_f64<kDppWaveShr1>(hl[o + NL], f[E + o]);
+            else W[t] = dpp_f64<kDppWaveRor1>(f[E + o], f[E + o]);
+        } else if constexpr (o < E) {
+            W[t] = f[o];
+        } else {
+            if constexpr (NW > 1) W[t] = dpp_f64<kDppWaveShl1>(hr[o - E], f[o - E]);
+            else W[t] = dpp_f64<kDppWaveRol1>(f[o - E], f[o - E]);
+        }
+    });
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        double acc;
+        if constexpr (sizeof(T) == 8) {
+            acc = c[e][0] * W[e];
+#pragma unroll
+            for (int t = 1; t < TAPS; ++t) acc = acc + c[e][t] * W[e + t];
+        } else {
+            acc = 0.;
+#pragma unroll
+            for (int t = 0; t < TAPS; ++t) acc = fma(c[e][t], W[e + t], acc);
+        }
+        g[e] = acc;
+    }
+}
+
+// ns steps of one line held in f.  Steps alternate between two register
+// sets (f -> g -> f), so no step copies its results back; the LDS exchange
+// record alternates halves with the step parity (one barrier per step).
+template <class T, int E, int NW, int TAPS, int DMIN>
+__device__ __forceinline__ void shfl_steps(double (&f)[E], const double (&c)[E][TAPS], int ns, double *xch, int w,
+                                           int lane) {
     const int wl = w == 0 ? NW - 1 : w - 1, wr = w == NW - 1 ? 0 : w + 1;
-    for (int s = 0; s < ns; ++s) {
-        double hl[NL > 0 ? NL : 1], hr[NR > 0 ? NR : 1];
-        if constexpr (NW > 1) {
-            // edge cells of the neighbouring waves (periodic over the line)
-            double *buf = xch + (s & 1) * (NW * 2 * E);
-            if (lane == 0) {
+    double *buf1 = xch + NW * 2 * E;
+    double g[E];
+    int s = 0;
+    for (; s + 2 <= ns; s += 2) {
+        shfl_step<T, E, NW, TAPS, DMIN>(f, g, c, xch, w, wl, wr, lane);
+        shfl_step<T, E, NW, TAPS, DMIN>(g, f, c, buf1, w, wl, wr, lane);
+    }
+    if (s < ns) {
+        shfl_step<T, E, NW, TAPS, DMIN>(f, g, c, xch, w, wl, wr, lane);
 #pragma unroll
-                for (int j = 0; j < NR; ++j) buf[(2 * w) * E + j] = f[j];
-            }
-            if (lane == 63) {
-#pragma unroll
-                for (int j = 0; j < NL; ++j) buf[(2 * w + 1) * E + E - NL + j] = f[E - NL + j];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < NL; ++j) hl[j] = buf[(2 * wl + 1) * E + E - NL + j];
-#pragma unroll
-            for (int j = 0; j < NR; ++j) hr[j] = buf[(2 * wr) * E + j];
-        }
-        // W[t] = the cell at offset DMIN + t from this lane's first cell
-        double W[E + TAPS - 1];
-        static_for<0, E + TAPS - 1>([&](auto tt) {
-            constexpr int t = decltype(tt)::value;
-            constexpr int o = DMIN + t;
-            if constexpr (o < 0) {
-                if constexpr (NW > 1) W[t] = dpp_f64<kDppWaveShr1>(hl[o + NL], f[E + o]);
-                else W[t] = dpp_f64<kDppWaveRor1>(f[E + o], f[E + o]);
-            } else if constexpr (o < E) {
-                W[t] = f[o];
-            } else {
-                if constexpr (NW > 1) W[t] = dpp_f64<kDppWaveShl1>(hr[o - E], f[o - E]);
-                else W[t] = dpp_f64<kDppWaveRol1>(f[o - E], f[o - E]);
-            }
-        });
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            double acc;
-            if constexpr (sizeof(T) == 8) {
-                acc = c[e][0] * W[e];
-#pragma unroll
-                for (int t = 1; t < TAPS; ++t) acc = acc + c[e][t] * W[e + t];
-            } else {
-                acc = 0.;
-#pragma unroll
-                for (int t = 0; t < TAPS; ++t) acc = fma(c[e][t], W[e + t], acc);
-            }
-            f[e] = acc;  // W holds copies: the other cells' windows are unaffected
-        }
+        for (int e = 0; e < E; ++e) f[e] = g[e];
     }
 }
 

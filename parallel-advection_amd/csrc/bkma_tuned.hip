@@ -751,14 +751,23 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
 // when every slab has at most two shifts (FP32 with the DPP broadcast at 5,
 // FP64 at 4 steps per pass), else the column stream with the smallest window (FP32 at 2, FP64 at
 // 3 steps per pass; more steps push the register window past the occupancy
-// that hides HBM latency), widest vector, deepest prefetch; else LINE_LDS
-// whole line, else the longest segment.
+// that hides HBM latency), widest vector, deepest prefetch; contiguous
+// lines that fit LINE_SHFL with 256 steps per pass, 8 cells per lane from
+// 4096 lines on (measured on [2^21][1024][1]), else 4 (on [256][1024][1]);
+// else LINE_LDS whole line, else the longest segment.
 int heuristic_score(const bkma_tuned_config &c, bkma_dtype dt, const bkma_adv_params &p) {
     if (c.strategy == BKMA_STRATEGY_PLAIN)  // short lines: one work-group per slab would idle
         return (p.n1 < 64 ? 900000000 : 10000000) - std::abs(c.seg_rows - 256);
     if (c.strategy == BKMA_STRATEGY_HYBRID) return 5000000 + c.chunk_segs;
-    if (c.strategy == BKMA_STRATEGY_LINE_SHFL)  // 8 cells per lane, 64 steps per pass
-        return 600000000 + c.vec * 1000 - std::abs(steps_per_pass(c) - 64);
+    if (c.strategy == BKMA_STRATEGY_LINE_SHFL) {
+        // 256 steps per pass; 8 cells per lane when the lines fill the GPU
+        // several times over, else 4 (twice the wavefronts)
+        const bool many = p.n0 >= 4096;
+        return 600000000 + ((c.vec == 8) == many ? 1000 : 0) - std::abs(steps_per_pass(c) - 256);
+    }
+    if ((c.strategy == BKMA_STRATEGY_COLUMN_PAIR || c.strategy == BKMA_STRATEGY_COLUMN_STREAM) &&
+        p.n2 < 64 * (u64)c.vec)  // a wavefront's lanes would mostly idle
+        return 1000 + c.vec;
     if (c.strategy == BKMA_STRATEGY_COLUMN_PAIR) {
         // FP32: DPP broadcast, FP64 stage-0 window, VEC 2, P 14, 5 steps;
         // FP64: VEC 2, P 14, 4 steps

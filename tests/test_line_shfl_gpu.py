@@ -207,3 +207,16 @@ def test_autotune_contiguous_lines(cuda, dtype):
     else:
         O.advect_f64(ref, po, 7)
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dtype", [B.F64, B.F32], ids=["f64", "f32"])
+def test_heuristic_plan_for_contiguous_lines(cuda, dtype):
+    # without the autotuner, contiguous lines go to LINE_SHFL where it fits and
+    # to LINE_LDS where it does not, never to the column kernels, whose 64
+    # lanes per wavefront span the n2 axis
+    for n1, prefix in ((1024, "line_shfl<"), (1000, "line_lds<")):
+        p = B.make_params(64, n1, 1, dt=1e-3, minRealVx=-1, maxRealVx=1)
+        plan = B.AdvPlan(p, dtype)
+        _, name = plan.describe()
+        plan.close()
+        assert name.startswith(prefix), (n1, name)
