@@ -51,6 +51,14 @@ template <int CTRL> __device__ __forceinline__ double dpp_f64(double old, double
     const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(s >> 32), CTRL, 0xF, 0xF, false);
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
+// DPP move of an FP64 register where every lane has a source lane
+// (rotations): no `old` operand, so no copy of it into the destination.
+template <int CTRL> __device__ __forceinline__ double dpp_rot_f64(double src) {
+    const long long v = __double_as_longlong(src);
+    const int lo = __builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
 constexpr int kDppWaveShl1 = 0x130;  // lane l reads lane l+1
 constexpr int kDppWaveRol1 = 0x134;  // lane l reads lane (l+1) mod 64
 constexpr int kDppWaveShr1 = 0x138;  // lane l reads lane l-1
@@ -90,12 +98,12 @@ __device__ __forceinline__ void shfl_step(const double (&f)[E], double (&g)[E], 
         constexpr int o = DMIN + t;
         if constexpr (o < 0) {
             if constexpr (NW > 1) W[t] = dpp_f64<kDppWaveShr1>(hl[o + NL], f[E + o]);
-            else W[t] = dpp_f64<kDppWaveRor1>(f[E + o], f[E + o]);
+            else W[t] = dpp_rot_f64<kDppWaveRor1>(f[E + o]);
         } else if constexpr (o < E) {
             W[t] = f[o];
         } else {
             if constexpr (NW > 1) W[t] = dpp_f64<kDppWaveShl1>(hr[o - E], f[o - E]);
-            else W[t] = dpp_f64<kDppWaveRol1>(f[o - E], f[o - E]);
+            else W[t] = dpp_rot_f64<kDppWaveRol1>(f[o - E]);
         }
     });
 #pragma unroll
@@ -217,11 +225,75 @@ __global__ __launch_bounds__(64 * NW) void adv_line_shfl_kernel(T *__restrict__ 
     }
 }
 
+// Lines of 16 cells: one line per 16-lane DPP row, one cell per lane, four
+// lines per wavefront.  Every cell's window covers the same offsets
+// WMIN .. WMIN+TW-1 (the union of the windows of all lines, TW <= 8); a
+// cell's six coefficients sit at its own offset in it and the other taps are
+// zero.  Neighbour o of lane r is lane (r + o) mod 16 of the row: one DPP
+// row_ror:((16 - o) mod 16) per tap and FP64 half.
+template <int O> constexpr int row_ror_ctrl() { return 0x120 + ((16 - (O % 16)) % 16); }
+
+template <class T, int WMIN, int TW>
+__global__ __launch_bounds__(256) void adv_line_row16_kernel(T *__restrict__ data, ShflArgs a) {
+    static_assert(TW >= 6 && TW <= 8, "window of 6 to 8 cells");
+    const AdvDev &p = a.p;
+    const int r = threadIdx.x & 15;
+    const i64 i0 = (i64)blockIdx.x * 16 + (threadIdx.x >> 4);
+    const bool valid = i0 < p.n0;  // whole rows are valid or not
+    T *cell = data + (valid ? i0 : 0) * 16 + r;
+    double f = valid ? (double)*cell : 0.0;
+
+    double c[TW];
+    {
+        double d_prev1;
+        const int leftNode = valid ? adv_foot(p, (int)i0, r, d_prev1) : r;
+        if (!valid) d_prev1 = 2.0;
+        double cf[6];
+        adv_lag_basis(d_prev1, cf);
+        int rel = leftNode - 2 - r;  // first tap's offset, normalised into (-8, 8]
+        if (rel > 8) rel -= 16;
+        if (rel <= -8) rel += 16;
+        const int k = valid ? rel - WMIN : 0;  // in [0, TW - 6] (checked at plan creation)
+#pragma unroll
+        for (int t = 0; t < TW; ++t) {
+            double v = 0.0;
+#pragma unroll
+            for (int j = 0; j <= TW - 6; ++j)
+                if (t - j >= 0 && t - j < 6 && k == j) v = cf[t - j];
+            c[t] = v;
+        }
+    }
+    for (int s = 0; s < a.ns; ++s) {
+        double W[TW];
+        static_for<0, TW>([&](auto tt) {
+            constexpr int t = decltype(tt)::value;
+            constexpr int o = WMIN + t;
+            if constexpr (o % 16 == 0) W[t] = f;
+            else W[t] = dpp_rot_f64<row_ror_ctrl<o>()>(f);
+        });
+        double acc;
+        if constexpr (sizeof(T) == 8) {
+            acc = c[0] * W[0];
+#pragma unroll
+            for (int t = 1; t < TW; ++t) acc = acc + c[t] * W[t];
+        } else {
+            acc = 0.;
+#pragma unroll
+            for (int t = 0; t < TW; ++t) acc = fma(c[t], W[t], acc);
+        }
+        f = acc;
+    }
+    if (valid) *cell = (T)f;
+}
+
 // Host side (line_shfl.hip): one pass of a.ns steps over n0 lines of
 // 64 * cells * nw cells; cells 4 with nw in {1, 2, 4, 8, 16}, cells 8 with
 // nw in {1, 2, 4, 8} (at 16 wavefronts per work-group a wave has 128 VGPRs,
 // fewer than 8 cells' cached coefficients need).
 bkma_status launch_line_shfl(bkma_dtype dt, int cells, int nw, void *data, const ShflArgs &a, i64 n0,
                              hipStream_t s);
+// Lines of 16 cells (adv_line_row16_kernel): window offsets wmin .. wmin+tw-1
+// with wmin in [-6, 0] and tw in [6, 8].
+bkma_status launch_line_row16(bkma_dtype dt, int wmin, int tw, void *data, const ShflArgs &a, i64 n0, hipStream_t s);
 
 }  // namespace bkma

@@ -8,9 +8,10 @@
 // two adjacent shifts (adv_pair.hpp): seven-tap windows, no shift branches.
 // Strategy LINE_LDS (any n2, any shift): slabs or segments of them staged in
 // LDS (adv_lds.hpp), up to 4 steps per pass.
-// Strategy LINE_SHFL (contiguous lines, n2 = 1, of 256 .. 4096 cells): each
-// line held in registers by one work-group for a pass of up to 256 steps,
-// stencil neighbours through DPP wavefront shifts (adv_shfl.hpp).
+// Strategy LINE_SHFL (contiguous lines, n2 = 1, of 256 .. 4096 cells, or of
+// 16): each line held in registers by one work-group (or one 16-lane row)
+// for a pass of up to 256 steps, stencil neighbours through DPP wavefront
+// shifts or row rotations (adv_shfl.hpp).
 //
 // FP64 storage: results bit-identical to the reference's FP64 steps.
 // FP32 storage: FP64 foot location / index / lag_basis exactly as the
@@ -121,6 +122,7 @@ struct TunedPlan {
     i64 persistent_blocks[9] = {};    // by steps per pass: 0 = one wave per unit, else the grid cap
     int *slab_shift = nullptr;         // COLUMN_PAIR, LINE_SHFL: smallest and largest shift of each slab, [n0][2] (owned)
     int shfl_waves = 0;                // LINE_SHFL: wavefronts per line
+    int row_wmin = 0, row_tw = 0;      // LINE_SHFL, 16-cell lines: window offsets [row_wmin, row_wmin + row_tw)
     bkma_optim_params optim{};         // PLAIN, HYBRID: the reference's dispatch for its AdaptiveWg executor
     // HYBRID: slabs [0, k_local) with their lines' scratch in LDS (optim),
     // [k_local, n0) with a global scratch (optim_global, gscratch), the two
@@ -271,15 +273,41 @@ bkma_status stream_feasible(const bkma_adv_params &p, bkma_dtype dt, const Strea
 // cells), every line's rows within two adjacent shifts, and every line's
 // window start -dmax - 2 in the range the lanes g-1 .. g+1 cover (stats: the
 // slab shift statistics; lines with two shifts need one cell more).
+// cells = 1: lines of 16 cells, one per 16-lane DPP row; every cell's window
+// covers the offsets of all lines, [*wmin, *wmin + *tw), with *wmin in
+// [-6, 0] and *tw at most 8.
 bkma_status shfl_feasible(const bkma_adv_params &p, int cells, int ns, const int stats[kShiftStats], int *nw,
-                          std::string *why) {
+                          std::string *why, int *wmin = nullptr, int *tw = nullptr) {
     if (!shift_supported(p, why)) return BKMA_ERR_UNSUPPORTED;
     if (p.n2 != 1) {
         if (why) *why = "LINE_SHFL needs contiguous lines (n2 = 1)";
         return BKMA_ERR_UNSUPPORTED;
     }
+    if (ns < 1 || ns > kShflMaxSteps) {
+        if (why) *why = "LINE_SHFL fuses 1 to 256 steps per pass";
+        return BKMA_ERR_UNSUPPORTED;
+    }
+    if (cells == 1) {
+        if (p.n1 != 16) {
+            if (why) *why = "LINE_SHFL with one cell per lane needs lines of 16 cells";
+            return BKMA_ERR_UNSUPPORTED;
+        }
+        if (stats[0] > stats[1]) {
+            if (why) *why = "no shift statistics";
+            return BKMA_ERR_UNSUPPORTED;
+        }
+        const int rlo = -stats[1] - 2, rhi = -stats[0] - 2;  // first-tap offsets of all cells
+        if (rlo < -6 || rlo > 0 || rhi - rlo > 2) {
+            if (why) *why = "16-cell lines: shifts beyond the compiled windows";
+            return BKMA_ERR_UNSUPPORTED;
+        }
+        if (wmin) *wmin = rlo;
+        if (tw) *tw = rhi - rlo + 6;
+        *nw = 4;
+        return BKMA_OK;
+    }
     if (cells != 4 && cells != 8) {
-        if (why) *why = "LINE_SHFL holds 4 or 8 cells per lane";
+        if (why) *why = "LINE_SHFL holds 1, 4 or 8 cells per lane";
         return BKMA_ERR_UNSUPPORTED;
     }
     const u64 per_wave = 64 * (u64)cells;
@@ -287,10 +315,6 @@ bkma_status shfl_feasible(const bkma_adv_params &p, int cells, int ns, const int
     if (p.n1 % per_wave != 0 || (w != 1 && w != 2 && w != 4 && w != 8 && !(w == 16 && cells == 4))) {
         if (why) *why = cells == 4 ? "LINE_SHFL with 4 cells per lane needs n1 = 256 .. 4096, a power of two"
                                    : "LINE_SHFL with 8 cells per lane needs n1 = 512 .. 4096, a power of two";
-        return BKMA_ERR_UNSUPPORTED;
-    }
-    if (ns < 1 || ns > kShflMaxSteps) {
-        if (why) *why = "LINE_SHFL fuses 1 to 256 steps per pass";
         return BKMA_ERR_UNSUPPORTED;
     }
     if (stats[2] > 1) {
@@ -424,6 +448,7 @@ bkma_status plan_launch(const TunedPlan &pl, void *data, hipStream_t s, int ns) 
         a.p = make_adv_dev(pl.params);
         a.slab_shift = pl.slab_shift;
         a.ns = ns;
+        if (pl.cfg.vec == 1) return launch_line_row16(pl.dtype, pl.row_wmin, pl.row_tw, data, a, (i64)pl.params.n0, s);
         return launch_line_shfl(pl.dtype, pl.cfg.vec, pl.shfl_waves, data, a, (i64)pl.params.n0, s);
     }
     if (ns < 1 || ns > pl.variant->nsmax) return fail(BKMA_ERR_RUNTIME, "tuned advection: steps per pass out of range");
@@ -604,14 +629,16 @@ bkma_status build_plan(const bkma_adv_params &p, bkma_dtype dt, const bkma_tuned
             return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
         const int cells = cfg.vec > 0 ? cfg.vec : 8;
         BKMA_TRY(scan_slab_shifts(p, &pl->slab_shift, stats));
-        if (shfl_feasible(p, cells, ns, stats, &pl->shfl_waves, &why) != BKMA_OK)
+        if (shfl_feasible(p, cells, ns, stats, &pl->shfl_waves, &why, &pl->row_wmin, &pl->row_tw) != BKMA_OK)
             return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: configuration infeasible (" + why + ")");
         pl->ns = ns;
         pl->cfg.steps_per_pass = ns;
         pl->cfg.vec = cells;
         pl->cfg.waves_per_wg = pl->shfl_waves;
-        pl->name = std::string("line_shfl<") + (dt == BKMA_F64 ? "f64" : "f32") + ",cells" + std::to_string(cells) +
-                   ",wg" + std::to_string(64 * pl->shfl_waves) + ",steps" + std::to_string(ns) + ">";
+        pl->name = std::string("line_shfl<") + (dt == BKMA_F64 ? "f64" : "f32") +
+                   (cells == 1 ? std::string(",row16")
+                               : ",cells" + std::to_string(cells) + ",wg" + std::to_string(64 * pl->shfl_waves)) +
+                   ",steps" + std::to_string(ns) + ">";
         return BKMA_OK;
     }
     return fail(BKMA_ERR_UNSUPPORTED, "tuned advection: unknown strategy");
@@ -658,7 +685,7 @@ std::vector<bkma_tuned_config> candidates(const bkma_adv_params &p, bkma_dtype d
         }
     }
     // contiguous lines in registers, neighbours through DPP
-    for (int cells : {8, 4}) {
+    for (int cells : {8, 4, 1}) {
         int nw = 0;
         if (shfl_feasible(p, cells, 1, stats, &nw, why) != BKMA_OK) continue;
         for (int ns : {1, 4, 16, 64, 256}) {

@@ -101,6 +101,22 @@ int probe_dpp_wave_move(const void *a, const void *old, void *out, unsigned long
     }
     return done("dpp_wave_move_kernel");
 }
+// a[64] u32 -> out[64]: DPP row_ror:r (r = 1 .. 15) within each 16-lane row
+int probe_dpp_row_ror(const void *a, void *out, unsigned long long r, void *stream) {
+    auto *pa = (const unsigned *)a;
+    auto *pr = (unsigned *)out;
+    auto st = (hipStream_t)stream;
+#define ROR_CASE(R)                                                                                      \
+    case R:                                                                                              \
+        hipLaunchKernelGGL(dpp_wave_move_kernel<0x120 + R>, dim3(1), dim3(64), 0, st, pa, pa, pr); break;
+    switch (r) {
+        ROR_CASE(1) ROR_CASE(2) ROR_CASE(3) ROR_CASE(4) ROR_CASE(5) ROR_CASE(6) ROR_CASE(7) ROR_CASE(8)
+        ROR_CASE(9) ROR_CASE(10) ROR_CASE(11) ROR_CASE(12) ROR_CASE(13) ROR_CASE(14) ROR_CASE(15)
+        default: return 1;
+    }
+#undef ROR_CASE
+    return done("dpp_wave_move_kernel");
+}
 // in[n] f32 (n even) -> out[n] e4m3 bytes
 int probe_fp8_convert(const void *in, void *out, unsigned long long n, void *stream) {
     if (n % 2) return 1;

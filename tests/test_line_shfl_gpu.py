@@ -58,6 +58,24 @@ def test_dpp_wave_moves(cuda, which, name):
     assert np.array_equal(o, exp), (o, exp)
 
 
+@pytest.mark.parametrize("r", list(range(1, 16)))
+def test_dpp_row_ror(cuda, r):
+    # row_ror:r: lane l of each 16-lane row reads lane (l - r) mod 16 of the same row, so the cell at
+    # offset o of a 16-cell line held one cell per lane arrives through row_ror:((16 - o) mod 16)
+    torch = _torch()
+    B.lib()
+    P = ctypes.CDLL(PROBES)
+    a = torch.arange(100, 164, dtype=torch.int32, device="cuda")
+    out = torch.empty(64, dtype=torch.int32, device="cuda")
+    st = P.probe_dpp_row_ror(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(out.data_ptr()), ctypes.c_ulonglong(r),
+                             ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert st == 0
+    torch.cuda.synchronize()
+    lanes = np.arange(64)
+    exp = a.cpu().numpy()[(lanes & ~15) | ((lanes - r) & 15)]
+    assert np.array_equal(out.cpu().numpy(), exp)
+
+
 def _shfl_config(ns, cells=8):
     c = B.TunedConfig()
     c.strategy = B.STRATEGY_LINE_SHFL
@@ -220,3 +238,46 @@ def test_heuristic_plan_for_contiguous_lines(cuda, dtype):
         _, name = plan.describe()
         plan.close()
         assert name.startswith(prefix), (n1, name)
+
+
+# Lines of 16 cells, one per 16-lane DPP row: (dt * n1, minRealVx, dv, n0, minRealX); v = minRealVx + i0 * dv.
+#  - the reference's e3 velocity law on 37 lines (two full 16-line groups and a partial one): s = 0.016 v,
+#    first-tap offsets -3 and -2;
+#  - s = v by halves on x in [0.1, 1.1) (integer shifts give shift pairs): offsets -1..0, -6..-4, -4..-2
+#    (a window of 8 cells), and 0 alone.
+ROW_CASES = [(0.016, -1.0, 2.0 / 37, 37, 0.0), (1.0, -2.5, 0.5, 3, 0.1), (1.0, 2.0, 0.5, 3, 0.1),
+             (1.0, -0.5, 0.5, 4, 0.1), (1.0, -2.5, 0.5, 1, 0.0)]
+
+
+@pytest.mark.parametrize("ns", [1, 3, 16])
+@pytest.mark.parametrize("case", ROW_CASES, ids=["e3_law", "offsets-1_0", "offsets-6_-4", "offsets-4_-2", "offset0"])
+@pytest.mark.parametrize("dtype", [B.F64, B.F32], ids=["f64", "f32"])
+def test_line_shfl_rows_bit_exact(cuda, dtype, case, ns):
+    dtn, vmin, dv, n0, x0 = case
+    n1 = 16
+    kw = dict(dt=dtn / n1, minRealVx=vmin, maxRealVx=vmin + dv * n0, minRealX=x0, maxRealX=x0 + 1)
+    p = B.make_params(n0, n1, 1, **kw)
+    po = O.make_params(n0, n1, 1, **kw)
+    seed = 41 + n0 + ns
+    if dtype == B.F32:
+        host = O.uniform(n0 * n1, seed, np.float32).reshape(n0, n1, 1)
+    else:
+        host = O.uniform(n0 * n1, seed).reshape(n0, n1, 1)
+    nsteps = 2 * ns + 1
+    got, cfg, name = _run(host, p, dtype, nsteps, _shfl_config(ns, 1))
+    assert name.startswith("line_shfl<") and ",row16," in name
+    ref = host.copy()
+    if dtype == B.F32:
+        O.advect_f32_blocked(ref, po, nsteps, ns)
+    else:
+        O.advect_f64(ref, po, nsteps)
+    bits = np.uint32 if dtype == B.F32 else np.uint64
+    assert np.array_equal(got.view(bits), ref.view(bits)), np.abs(got - ref).max()
+
+
+@pytest.mark.parametrize("n1,vmin,dv,n0", [(16, -1.5, 1.0, 4), (32, -0.5, 0.5, 3), (16, 3.5, 0.5, 2)],
+                         ids=["offsets_spread_3", "n1_32", "shift_beyond_window"])
+def test_line_shfl_rows_rejections(cuda, n1, vmin, dv, n0):
+    p = B.make_params(n0, n1, 1, dt=1.0 / n1, minRealVx=vmin, maxRealVx=vmin + dv * n0, minRealX=0.1, maxRealX=1.1)
+    with pytest.raises(B.BkmaUnsupported):
+        B.AdvPlan(p, B.F64, config=_shfl_config(4, 1))
