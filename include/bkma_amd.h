@@ -213,9 +213,9 @@ typedef enum bkma_strategy {
                                         a global scratch, concurrently on a second stream (F64 storage;
                                         seg_rows = pref_wg_size), one step per pass */
     BKMA_STRATEGY_LINE_SHFL = 6      /* contiguous lines (n2 = 1) held in registers for up to 256 steps per pass:
-                                        lines of 256 .. 4096 cells (a power of two) whose rows take at most two
-                                        adjacent shifts, one work-group per line (vec = 4 or 8 cells per lane,
-                                        0 = 8; waves_per_wg = n1 / (64 vec), at most 8 with 8 cells), neighbours
+                                        lines of 64 vec k cells (k = 1 .. 16, at most 4096 cells) whose rows take
+                                        at most two adjacent shifts, one work-group per line (vec = 4 or 8 cells
+                                        per lane, 0 = 8; waves_per_wg = k, at most 8 with 8 cells), neighbours
                                         through DPP wavefront shifts, shifts up to about vec - 3 cells; or lines
                                         of 16 cells (vec = 1), one per 16-lane row, neighbours through DPP row
                                         rotations, all lines' first-tap offsets within 3 consecutive values */

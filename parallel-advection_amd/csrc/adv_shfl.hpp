@@ -287,8 +287,8 @@ __global__ __launch_bounds__(256) void adv_line_row16_kernel(T *__restrict__ dat
 }
 
 // Host side (line_shfl.hip): one pass of a.ns steps over n0 lines of
-// 64 * cells * nw cells; cells 4 with nw in {1, 2, 4, 8, 16}, cells 8 with
-// nw in {1, 2, 4, 8} (at 16 wavefronts per work-group a wave has 128 VGPRs,
+// 64 * cells * nw cells; cells 4 with nw in 1 .. 16, cells 8 with nw in
+// 1 .. 8 (from 9 wavefronts per work-group on, a wave has at most 128 VGPRs,
 // fewer than 8 cells' cached coefficients need).
 bkma_status launch_line_shfl(bkma_dtype dt, int cells, int nw, void *data, const ShflArgs &a, i64 n0,
                              hipStream_t s);

@@ -8,8 +8,8 @@
 // two adjacent shifts (adv_pair.hpp): seven-tap windows, no shift branches.
 // Strategy LINE_LDS (any n2, any shift): slabs or segments of them staged in
 // LDS (adv_lds.hpp), up to 4 steps per pass.
-// Strategy LINE_SHFL (contiguous lines, n2 = 1, of 256 .. 4096 cells, or of
-// 16): each line held in registers by one work-group (or one 16-lane row)
+// Strategy LINE_SHFL (contiguous lines, n2 = 1, of 256 k <= 4096 cells, or
+// of 16): each line held in registers by one work-group (or one 16-lane row)
 // for a pass of up to 256 steps, stencil neighbours through DPP wavefront
 // shifts or row rotations (adv_shfl.hpp).
 //
@@ -269,7 +269,7 @@ bkma_status stream_feasible(const bkma_adv_params &p, bkma_dtype dt, const Strea
 }
 
 // LINE_SHFL feasibility for `cells` consecutive cells per lane: contiguous
-// lines of 64 * cells * NW cells (NW = 1, 2, 4 or 8 wavefronts; 16 with 4
+// lines of 64 * cells * NW cells (NW = 1 .. 8 wavefronts; 1 .. 16 with 4
 // cells), every line's rows within two adjacent shifts, and every line's
 // window start -dmax - 2 in the range the lanes g-1 .. g+1 cover (stats: the
 // slab shift statistics; lines with two shifts need one cell more).
@@ -312,9 +312,9 @@ bkma_status shfl_feasible(const bkma_adv_params &p, int cells, int ns, const int
     }
     const u64 per_wave = 64 * (u64)cells;
     const u64 w = p.n1 / per_wave;
-    if (p.n1 % per_wave != 0 || (w != 1 && w != 2 && w != 4 && w != 8 && !(w == 16 && cells == 4))) {
-        if (why) *why = cells == 4 ? "LINE_SHFL with 4 cells per lane needs n1 = 256 .. 4096, a power of two"
-                                   : "LINE_SHFL with 8 cells per lane needs n1 = 512 .. 4096, a power of two";
+    if (p.n1 % per_wave != 0 || w < 1 || w > (cells == 4 ? 16u : 8u)) {
+        if (why) *why = cells == 4 ? "LINE_SHFL with 4 cells per lane needs n1 = 256 k, k = 1 .. 16"
+                                   : "LINE_SHFL with 8 cells per lane needs n1 = 512 k, k = 1 .. 8";
         return BKMA_ERR_UNSUPPORTED;
     }
     if (stats[2] > 1) {

@@ -12,18 +12,20 @@ bkma_status launch_nw(void *data, const ShflArgs &a, i64 n0, hipStream_t s) {
     return check_launch("adv_line_shfl_kernel");
 }
 
+// Wavefronts per line: 1 .. 16 with 4 cells per lane, 1 .. 8 with 8.
+template <class T, int E, int NW>
+bkma_status launch_nw_from(int nw, void *data, const ShflArgs &a, i64 n0, hipStream_t s) {
+    if constexpr (NW > (E == 4 ? 16 : 8)) {
+        return fail(BKMA_ERR_UNSUPPORTED, "LINE_SHFL: wavefronts per line not compiled");
+    } else {
+        if (nw == NW) return launch_nw<T, E, NW>(data, a, n0, s);
+        return launch_nw_from<T, E, NW + 1>(nw, data, a, n0, s);
+    }
+}
+
 template <class T, int E>
 bkma_status launch_e(int nw, void *data, const ShflArgs &a, i64 n0, hipStream_t s) {
-    switch (nw) {
-        case 1: return launch_nw<T, E, 1>(data, a, n0, s);
-        case 2: return launch_nw<T, E, 2>(data, a, n0, s);
-        case 4: return launch_nw<T, E, 4>(data, a, n0, s);
-        case 8: return launch_nw<T, E, 8>(data, a, n0, s);
-        case 16:
-            if constexpr (E == 4) return launch_nw<T, E, 16>(data, a, n0, s);
-            [[fallthrough]];
-        default: return fail(BKMA_ERR_UNSUPPORTED, "LINE_SHFL: wavefronts per line not compiled");
-    }
+    return launch_nw_from<T, E, 1>(nw, data, a, n0, s);
 }
 
 template <class T>

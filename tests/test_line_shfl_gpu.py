@@ -8,7 +8,8 @@ oracle's restatement of the tuned FP32 arithmetic with the plan's steps per
 pass (orc_advect_f32_blocked).  The cases reach every window start the kernel
 compiles for 4 and 8 cells per lane (six-cell windows on lines with one
 shift, seven-cell windows on lines with a shift pair), every wavefront count
-per line, and a step count that leaves a shorter last pass."""
+per line (1 to 16, powers of two or not), and a step count that leaves a
+shorter last pass."""
 import ctypes
 import os
 
@@ -124,7 +125,7 @@ SHIFT_CASES = {8: [(1.0, -4.5, 6.0, 21, 0.1), (5.3, -0.93, 1.0, 7, 0.0)],
                4: [(1.0, -1.5, 2.0, 7, 0.1), (1.6, -0.93, 1.0, 7, 0.0)]}
 EXPECTED_SETS = {8: ({(d,) for d in range(-4, 7)}, {(d, d + 1) for d in range(-4, 6)}),
                  4: ({(d,) for d in range(-1, 3)}, {(d, d + 1) for d in range(-1, 2)})}
-LINES = {8: [512, 1024, 2048, 4096], 4: [256, 512, 1024, 2048, 4096]}
+LINES = {8: [512, 1024, 1536, 2048, 4096], 4: [256, 512, 768, 1024, 2048, 3840, 4096]}
 BIT_EXACT_CASES = [(cells, k, n1) for cells in (8, 4) for k in (0, 1) for n1 in LINES[cells]]
 
 
@@ -178,13 +179,14 @@ def test_line_shfl_c0_workload_bit_exact(cuda, cells):
     ((4, 1024, 2), 1.0, 0.0, 8),   # lines strided by n2 = 2
     ((4, 1000, 1), 1.0, 0.0, 8),   # n1 not a power of two
     ((4, 8192, 1), 1.0, 0.0, 8),   # 16 wavefronts of 8 cells per line
+    ((4, 4608, 1), 1.0, 0.0, 8),   # 9 wavefronts of 8 cells per line
     ((4, 8192, 1), 1.0, 0.0, 4),   # 32 wavefronts of 4 cells per line
     ((4, 128, 1), 1.0, 0.0, 4),    # half a wavefront per line
     ((4, 1024, 1), 1.0, 0.0, 6),   # 6 cells per lane
     ((2, 1024, 1), 7.5, 1.0, 8),   # shifts 7.5 and 9.375: window starts -10 and -12
     ((2, 1024, 1), 2.5, 1.0, 4),   # shifts 2.5 and 3.125: window starts -5 and -6
     ((2, 1024, 1), 2.0, -1.0, 4),  # shift -2: window start 0, past the lane g+1
-], ids=["strided", "ragged_n1", "n1_8192", "n1_8192_cells4", "n1_128", "cells6", "large_shift",
+], ids=["strided", "ragged_n1", "n1_8192", "n1_4608", "n1_8192_cells4", "n1_128", "cells6", "large_shift",
         "large_shift_cells4", "negative_shift_cells4"])
 def test_line_shfl_rejections(cuda, shape, dtn, vmin, cells):
     n0, n1, n2 = shape
