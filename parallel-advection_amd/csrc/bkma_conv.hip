@@ -1271,8 +1271,8 @@ template <class TIO, int C, int K, int TN> int occupancy_fp8(u64 lds_bytes) {
 
 #define FV(TIO, IO, C, K, TN) {IO, 0, C, K, TN, &launch_fp8<TIO, C, K, TN>, &occupancy_fp8<TIO, C, K, TN>}
 const MfmaVariant kFp8Variants[] = {
-    FV(float, BKMA_F32, 256, 7, 128),
-    FV(__hip_bfloat16, BKMA_BF16, 256, 7, 128),
+    FV(float, BKMA_F32, 256, 7, 256),
+    FV(__hip_bfloat16, BKMA_BF16, 256, 7, 256),
     FV(float, BKMA_F32, 128, 5, 128),
 };
 #undef FV
